@@ -1,0 +1,187 @@
+/*
+ * rubix_amd: C ABI of the MI355X (gfx950) particle-to-datacube path.
+ *
+ * Plain pointers and sizes only.  All array pointers are DEVICE pointers;
+ * every call is asynchronous on the given HIP stream (hipStream_t passed as
+ * void*; NULL = default stream) and returns 0 on success or a negative
+ * RBX_E* code; rbx_last_error_string() describes the last failure of the
+ * calling thread.  The library never allocates device memory: callers pass
+ * workspaces sized by the *_workspace_size queries.
+ *
+ * Layouts (all row-major, float32 unless stated):
+ *   coords, velocity  (n, stride) - component c of star i at [i*stride + c]
+ *   flux              (n_met, n_age, n_wave)      rubix/spectra/ssp/grid.py:31
+ *   datacube          (nb, nb, n_target): spaxel id p = x + nb*y is row p,
+ *                     lambda contiguous            rubix/spectra/ifu.py:286-287
+ *
+ * Each entry point names the reference function it replaces (paths relative
+ * to the AstroAI-Lab/rubix tree).
+ */
+#ifndef RUBIX_AMD_H
+#define RUBIX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBX_OK 0
+#define RBX_EINVAL (-1)    /* bad argument */
+#define RBX_EHIP (-2)      /* HIP runtime error (launch / memset) */
+#define RBX_EWORKSPACE (-3) /* workspace too small or misaligned */
+
+/* Version (major*10000 + minor*100 + patch) and diagnostics. */
+int rbx_version(void);
+const char* rbx_last_error_string(void);
+
+/* ---------------------------------------------------------------- a4 / a6
+ * Spaxel assignment.  Replaces square_spaxel_assignment
+ * (rubix/telescope/utils.py:77-152) as called by the spaxel_assignment step
+ * (rubix/core/telescope.py:108-124):
+ *   ix = clip(searchsorted(edges, x, 'right') - 1, 0, nbins - 1), same for y,
+ *   out = ix + nbins * iy, nbins = n_edges - 1; NaN sorts after every edge.
+ * Coordinates x, y are components 0 and 1 of each star's row.
+ */
+int rbx_spaxel_assign(const float* coords, int64_t n, int64_t stride,
+                      const float* edges, int32_t n_edges, int32_t* out_ids,
+                      void* stream);
+
+/* Stable sort-by-spaxel index (new; the reference analogue is the lexsort +
+ * searchsorted at rubix/spectra/dust/dust_extinction.py:237-267).
+ *   perm     (n)          stable argsort of ids (ids outside [0, n_keys) last,
+ *                         in input order)
+ *   sorted   (n)          ids[perm] (may be NULL)
+ *   offsets  (n_keys + 1) offsets[s] = first sorted position with id >= s,
+ *                         offsets[n_keys] = number of in-range ids.
+ * Deterministic (no floating point; integer atomics only on per-tile
+ * counters owned by one wavefront).
+ */
+size_t rbx_spaxel_sort_workspace_size(int64_t n, int32_t n_keys);
+int rbx_spaxel_sort(const int32_t* ids, int64_t n, int32_t n_keys,
+                    int32_t* perm, int32_t* sorted, int32_t* offsets,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------- SSP grid
+ * Template grid + the tables the fused kernels use.  wave_obs is the SSP
+ * wavelength already shifted by the cosmological (1+z) factor, in float32,
+ * as rubix/core/ifu.py:261 computes it.  log_axes != 0 means ax_met/ax_age
+ * hold log10 values and queries are log10'd before interpolation.
+ * rbx_ssp_prepare fills cum (double, n_met*n_age*n_wave: per-template
+ * prefix sums of flux[k]*(wave_obs[k]-wave_obs[k-1]), 0 at k = 0) and
+ * locate (int32, n_locate: last knot index <= locate_x0 + m/locate_inv_h).
+ */
+typedef struct rbx_ssp {
+  const float* flux;
+  const float* ax_met;
+  const float* ax_age;
+  const float* wave_obs;
+  const double* cum;
+  const int32_t* locate;
+  int32_t n_met, n_age, n_wave, n_locate;
+  float locate_x0, locate_inv_h;
+  int32_t log_axes;
+  int32_t reserved;
+} rbx_ssp;
+
+int rbx_ssp_prepare(const rbx_ssp* ssp, double* cum_out, int32_t* locate_out,
+                    void* stream);
+
+/* Telescope wavelength axis (rubix/core/ifu.py:254-255).  dwave is
+ * calculate_diff(wave) (first element 0, rubix/spectra/ifu.py:83-102);
+ * wmin/wmax are min/max of wave. */
+typedef struct rbx_target {
+  const float* wave;
+  const float* dwave;
+  int32_t n;
+  float wmin, wmax;
+} rbx_target;
+
+/* Per-star inputs of the fused path (pointers to (n,) arrays unless noted).
+ * velocity is (n, vel_stride); the line-of-sight component vel_comp is used
+ * (rubix/config/rubix_config.yml:245: "z" -> 2).  ids are spaxel ids from
+ * rbx_spaxel_assign; n_spaxel = nb*nb segments are kept, ids >= n_spaxel are
+ * dropped as segment_sum does (rubix/spectra/ifu.py:286); n_keys bounds the
+ * ids (nbins^2 of the edge array). */
+typedef struct rbx_stars {
+  int64_t n;
+  const float* mass;
+  const float* age;
+  const float* metallicity;
+  const float* velocity;
+  int64_t vel_stride;
+  int32_t vel_comp;
+  int32_t n_spaxel;
+  int32_t n_keys;
+  int32_t reserved;
+  const int32_t* ids;
+  float speed_of_light;
+  float reserved_f;
+} rbx_stars;
+
+/* ---------------------------------------------------------------- fused cube
+ * Forward: replaces the calculate_spectra -> scale_spectrum_by_mass ->
+ * doppler_shift_and_resampling -> calculate_datacube chain
+ * (rubix/core/ifu.py:63-121, 149-157, 269-293, 327-339;
+ *  rubix/spectra/ifu.py:190, 223-266, 269-288; interp2d linear at
+ *  rubix/spectra/ssp/grid.py:113-120) with one fused, atomic-free,
+ * deterministic path.  Writes cube (n_spaxel, target.n).  The workspace
+ * keeps the per-star state that rbx_cube_bwd reads: keep it untouched
+ * between the two calls.  Per-star spectra are never materialised.
+ */
+size_t rbx_cube_workspace_size(int64_t n, int32_t n_keys, int32_t n_target);
+int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target,
+                 const rbx_stars* stars, float* cube, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+/* Adjoint of rbx_cube_fwd (the reference has none:
+ * rubix/core/pipeline.py:203-208).  dcube (n_spaxel, target.n) is dL/dcube;
+ * writes dL/dmass, dL/dage, dL/dmetallicity (n,) and dL/dv_los (n,) - the
+ * gradient w.r.t. velocity[:, vel_comp].  Any output pointer may be NULL.
+ * Recomputes per-star spectra on the fly (no N x n_wave storage). */
+int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target,
+                 const rbx_stars* stars, const float* dcube, float* dmass,
+                 float* dage, float* dmet, float* dvlos, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------- PSF / LSF
+ * 1D convolution along the middle axis of an (n_outer, n_axis, n_inner)
+ * array with zero padding and a crop start:
+ *   out[o, x, i] = sum_t taps[t] * in[o, x + start - t, i].
+ * With start = (len-1)//2 this is one axis of convolve2d(mode="same")
+ * (rubix/telescope/psf/psf.py:9-11); with start = extend_factor it is the
+ * LSF full-convolution + crop (rubix/telescope/lsf/lsf.py:29-65).  The
+ * adjoint is the same call with reversed taps and start' = len-1-start.
+ */
+int rbx_conv_axis(const float* in, float* out, int64_t n_outer, int32_t n_axis,
+                  int64_t n_inner, const float* taps, int32_t n_taps,
+                  int32_t start, void* stream);
+
+/* Separable PSF: out = conv_y(conv_x(in)) over a (ny, nx, n_lambda) cube;
+ * tmp (same size as in) receives conv_x(in).  Replaces apply_psf
+ * (rubix/telescope/psf/psf.py:33-58) for kernels K = outer(gy, gx), which
+ * is exact for gaussian_kernel_2d (rubix/telescope/psf/kernels.py:5-31). */
+int rbx_psf_sep(const float* in, float* out, float* tmp, int32_t ny, int32_t nx,
+                int32_t nl, const float* gy, int32_t my, int32_t start_y,
+                const float* gx, int32_t mx, int32_t start_x, void* stream);
+
+/* General (non-separable) 2D kernel, per lambda plane, mode "same":
+ *   out[y, x, l] = sum_ab K[a, b] * in[y + sy - a, x + sx - b, l]. */
+int rbx_conv2d_planes(const float* in, float* out, int32_t ny, int32_t nx,
+                      int32_t nl, const float* kernel, int32_t ky, int32_t kx,
+                      int32_t sy, int32_t sx, void* stream);
+
+/* dot = sum(a * conv_axis(b)) in double with a fixed reduction order (used
+ * for dL/dsigma of the PSF and LSF).  partial: workspace of
+ * rbx_dot_workspace_size(total) bytes; result: one device double. */
+size_t rbx_dot_workspace_size(int64_t total);
+int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer,
+                      int32_t n_axis, int64_t n_inner, const float* taps,
+                      int32_t n_taps, int32_t start, double* result,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RUBIX_AMD_H */

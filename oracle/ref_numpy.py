@@ -1,0 +1,377 @@
+"""numpy restatement of rubix's particle -> datacube path (forward).
+
+TEST INFRASTRUCTURE ONLY: used by ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU baseline as the checker.  Never imported by ``rubix_amd``.
+
+Every function follows the reference op for op and cites the file:line it
+restates (paths relative to the AstroAI-Lab/rubix checkout).  ``dtype``
+selects the arithmetic: ``np.float32`` mirrors the reference (which never
+enables x64; the SSP grid is cast to float32 at rubix/spectra/ssp/grid.py:147),
+``np.float64`` is the high-precision oracle that errors are reported against.
+
+Pinning (what checks each function, in tests/test_oracle.py):
+* square_spaxel_assignment - reference known answer [4, 14]
+  (tests/test_telescope_utils.py:20-35) + brute-force loop + NaN/edge cases.
+* sort_by_spaxel           - np.argsort(kind="stable") identity + invariants.
+* lookup_linear            - reference lookup known answers
+  (tests/test_ssp_grid.py:653-698: node exactness, 0 outside, range bound),
+  brute-force bilinear on tiny grids, reproduction of affine functions.
+* velocity/cosmological doppler, calculate_diff - reference known answers
+  (tests/test_spectra_ifu.py:29-50,150-172).
+* interp_jax               - np.interp (library routine) incl. clamping.
+* resample_spectrum        - reference 5-point recipe and zero spectrum
+  (tests/test_spectra_ifu.py:175-228), flux conservation identity.
+* calculate_cube           - reference known answer
+  (tests/test_spectra_ifu.py:231-257) + brute force.
+* apply_psf                - scipy.signal.convolve2d(mode="same") on random
+  planes and kernels (odd sizes) + the reference delta/ones-kernel cases.
+* apply_lsf                - np.convolve(mode="full") + crop, and the
+  reference delta -> normalised Gaussian known answer
+  (tests/test_telescope_lsf.py:6-60).
+* gaussian_kernel_2d       - sums to 1, centre max, equals outer(g, g).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# rubix/config/rubix_config.yml:8
+SPEED_OF_LIGHT = 299792.458
+
+
+# ---------------------------------------------------------------- geometry
+
+def spatial_bin_edges(ang_scale, fov, spatial_bins, dtype=np.float32):
+    """rubix/telescope/utils.py:11-37 with the angular scale given.
+
+    aperture = ang_scale * fov; s = aperture / bins;
+    edges = arange(-aperture/2, aperture/2 + s, s).  The float arange can
+    yield bins+1 or bins+2 edges depending on rounding (a reference quirk
+    that the synthetic configs avoid by passing explicit edges).
+    """
+    a = dtype(ang_scale) * dtype(fov)
+    s = a / dtype(spatial_bins)
+    return np.arange(-a / 2, a / 2 + s, s, dtype=dtype), s
+
+
+def square_spaxel_assignment(coords, edges):
+    """rubix/telescope/utils.py:77-152.
+
+    ix = digitize(x, edges) - 1 (== searchsorted(edges, x, 'right') - 1),
+    clipped to [0, nbins-1] with nbins = len(edges) - 1; same for y;
+    id = ix + nbins * iy as int32.
+    """
+    coords = np.asarray(coords)
+    edges = np.asarray(edges)
+    nb = len(edges) - 1
+    ix = np.digitize(coords[:, 0], edges) - 1
+    iy = np.digitize(coords[:, 1], edges) - 1
+    ix = np.clip(ix, 0, nb - 1)
+    iy = np.clip(iy, 0, nb - 1)
+    return (ix + nb * iy).astype(np.int32)
+
+
+def mask_particles_outside_aperture(coords, edges):
+    """rubix/telescope/utils.py:155-178 (boundary inclusive)."""
+    lo, hi = np.min(edges), np.max(edges)
+    m = (coords[:, 0] >= lo) & (coords[:, 0] <= hi)
+    m &= (coords[:, 1] >= lo) & (coords[:, 1] <= hi)
+    return m
+
+
+def sort_by_spaxel(ids, n_seg):
+    """Stable sort-by-spaxel index (new in the build; analogue of the
+    lexsort + searchsorted(side='left') at
+    rubix/spectra/dust/dust_extinction.py:237-267).
+
+    Returns perm (stable argsort of ids) and offsets[s] = first sorted
+    position with id >= s, for s in [0, n_seg]; ids >= n_seg sort last
+    (they are the stars segment_sum drops).
+    """
+    ids = np.asarray(ids)
+    perm = np.argsort(ids, kind="stable").astype(np.int32)
+    srt = ids[perm]
+    offsets = np.searchsorted(srt, np.arange(n_seg + 1), side="left").astype(np.int32)
+    return perm, offsets
+
+
+def reshape_array(arr, n_dev=1):
+    """rubix/core/data.py:446-487: zero-pad the leading axis to a multiple
+    of n_dev and reshape to (n_dev, ceil(N/n_dev), ...)."""
+    arr = np.asarray(arr)
+    n = arr.shape[0]
+    per = (n + n_dev - 1) // n_dev
+    tot = per * n_dev
+    if tot > n:
+        pad = [(0, tot - n)] + [(0, 0)] * (arr.ndim - 1)
+        arr = np.pad(arr, pad, mode="constant")
+    return arr.reshape(n_dev, per, *arr.shape[1:])
+
+
+# ---------------------------------------------------------------- spectra
+
+def lookup_linear(metallicity, age, ax_met, ax_age, flux, log_axes=False,
+                  dtype=np.float32):
+    """Bilinear SSP lookup, ``interp2d(method="linear", extrap=0)`` as bound
+    at rubix/spectra/ssp/grid.py:113-120 (x = metallicity, y = age,
+    f = flux with layout (n_met, n_age, n_wave), grid.py:31).
+
+    Cell: i = clip(searchsorted(x, xq, 'right'), 1, nx-1) (same for y);
+    value = [(x2-xq)(y2-yq) f[i-1,j-1] + (xq-x1)(y2-yq) f[i,j-1]
+             + (x2-xq)(yq-y1) f[i-1,j] + (xq-x1)(yq-y1) f[i,j]] / (dx dy);
+    exactly 0 where xq or yq lies strictly outside its axis range.
+    ``log_axes``: the axes hold log10 values and the queries are log10'd
+    first (the bilinear-in-(log Z, log age) parity mode).
+    Returns (n_q, n_wave).
+    """
+    Z = np.asarray(metallicity, dtype=dtype)
+    t = np.asarray(age, dtype=dtype)
+    x = np.asarray(ax_met, dtype=dtype)
+    y = np.asarray(ax_age, dtype=dtype)
+    f = np.asarray(flux, dtype=dtype)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return _lookup_linear(Z, t, x, y, f, log_axes, dtype)
+
+
+def _lookup_linear(Z, t, x, y, f, log_axes, dtype):
+    xq = np.log10(Z) if log_axes else Z
+    yq = np.log10(t) if log_axes else t
+    i = np.clip(np.searchsorted(x, xq, side="right"), 1, len(x) - 1)
+    j = np.clip(np.searchsorted(y, yq, side="right"), 1, len(y) - 1)
+    x1, x2 = x[i - 1], x[i]
+    y1, y2 = y[j - 1], y[j]
+    ax2 = (x2 - xq)[:, None]
+    ax1 = (xq - x1)[:, None]
+    ay2 = (y2 - yq)[:, None]
+    ay1 = (yq - y1)[:, None]
+    num = (ax2 * ay2 * f[i - 1, j - 1] + ax1 * ay2 * f[i, j - 1]
+           + ax2 * ay1 * f[i - 1, j] + ax1 * ay1 * f[i, j])
+    den = ((x2 - x1) * (y2 - y1))[:, None]
+    out = num / den
+    outside = (xq < x[0]) | (xq > x[-1]) | (yq < y[0]) | (yq > y[-1])
+    out = np.where(outside[:, None], dtype(0), out).astype(dtype)
+    return out
+
+
+def scale_spectrum_by_mass(spectra, mass):
+    """rubix/core/ifu.py:149-157: spectra * mass[..., None]."""
+    return spectra * np.expand_dims(mass, -1).astype(spectra.dtype)
+
+
+def cosmological_doppler_shift(z, wavelength, dtype=np.float32):
+    """rubix/spectra/ifu.py:66-80: (1 + z) * wavelength (python-float
+    factor, evaluated in the array's float type)."""
+    return (dtype(1 + z) * np.asarray(wavelength, dtype=dtype)).astype(dtype)
+
+
+def velocity_doppler_shift(wavelength, velocity, direction="z",
+                           c=SPEED_OF_LIGHT, dtype=np.float32):
+    """rubix/spectra/ifu.py:168-220: w[i, k] = wavelength[k] * exp(v_i / c)
+    for the chosen velocity component (rubix_config.yml:245 uses "z")."""
+    comp = {"x": 0, "y": 1, "z": 2}
+    if direction not in comp:
+        raise ValueError(
+            f"{direction} is not a valid direction. Supported directions are 'x', 'y', or 'z'.")
+    v = np.asarray(velocity, dtype=dtype)[:, comp[direction]]
+    a = np.exp(v / dtype(c)).astype(dtype)
+    return (np.asarray(wavelength, dtype=dtype)[None, :] * a[:, None]).astype(dtype)
+
+
+def calculate_diff(vec, pad_with_zero=True):
+    """rubix/spectra/ifu.py:83-102: diff with the first element prepended
+    (so the first difference is 0)."""
+    vec = np.asarray(vec)
+    if pad_with_zero:
+        return np.diff(vec, prepend=vec[..., :1], axis=-1)
+    return np.diff(vec, axis=-1)
+
+
+def interp_jax(x, xp, fp):
+    """``jnp.interp`` semantics (used at rubix/spectra/ifu.py:250):
+    i = clip(searchsorted(xp, x, 'right'), 1, n-1);
+    f = fp[i-1] + (x - xp[i-1]) / (xp[i] - xp[i-1]) * (fp[i] - fp[i-1]);
+    f = fp[0] where x < xp[0]; f = fp[-1] where x > xp[-1]."""
+    x = np.asarray(x)
+    xp = np.asarray(xp)
+    fp = np.asarray(fp)
+    n = len(xp)
+    i = np.clip(np.searchsorted(xp, x, side="right"), 1, n - 1)
+    df = fp[i] - fp[i - 1]
+    dx = xp[i] - xp[i - 1]
+    delta = x - xp[i - 1]
+    eps = np.spacing(np.finfo(xp.dtype).eps)
+    dx0 = np.abs(dx) <= eps
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f = np.where(dx0, fp[i - 1], fp[i - 1] + (delta / np.where(dx0, 1, dx)) * df)
+    f = np.where(x < xp[0], fp[0], f)
+    f = np.where(x > xp[-1], fp[-1], f)
+    return f.astype(fp.dtype)
+
+
+def nan_to_num(x):
+    """``jnp.nan_to_num(x, nan=0.0)``: nan -> 0, +-inf -> +-finfo.max."""
+    return np.nan_to_num(x, nan=0.0)
+
+
+def resample_spectrum(initial_spectrum, initial_wavelength, target_wavelength):
+    """rubix/spectra/ifu.py:223-266 for one spectrum.
+
+    mask = (w >= min(target)) & (w <= max(target));
+    T = sum(S * diff(w) * mask); P = interp(target, w, S);
+    Nn = sum(P * diff(target)); out = P * nan_to_num(T / Nn)."""
+    S = np.asarray(initial_spectrum)
+    w = np.asarray(initial_wavelength)
+    tw = np.asarray(target_wavelength)
+    m = (w >= np.min(tw)) & (w <= np.max(tw))
+    iwd = calculate_diff(w) * m
+    total = np.sum(S * iwd, dtype=S.dtype)
+    P = interp_jax(tw, w, S)
+    new_total = np.sum(P * calculate_diff(tw), dtype=S.dtype)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sf = S.dtype.type(total) / S.dtype.type(new_total)
+    sf = nan_to_num(sf)
+    return (P * sf).astype(S.dtype)
+
+
+def resample_spectra(spectra, wavelengths, target_wavelength):
+    """vmap of resample_spectrum over stars (rubix/core/ifu.py:163-199)."""
+    out = np.empty((spectra.shape[0], len(target_wavelength)), dtype=spectra.dtype)
+    for i in range(spectra.shape[0]):
+        out[i] = resample_spectrum(spectra[i], wavelengths[i], target_wavelength)
+    return out
+
+
+def calculate_cube(spectra, spaxel_index, num_spaxels):
+    """rubix/spectra/ifu.py:269-288: segment_sum over spaxel ids into
+    num_spaxels**2 segments (ids outside [0, nb^2) dropped), reshaped to
+    (nb, nb, n_wave)."""
+    spectra = np.asarray(spectra)
+    ids = np.asarray(spaxel_index)
+    ns = num_spaxels * num_spaxels
+    out = np.zeros((ns, spectra.shape[-1]), dtype=spectra.dtype)
+    ok = (ids >= 0) & (ids < ns)
+    np.add.at(out, ids[ok], spectra[ok])
+    return out.reshape(num_spaxels, num_spaxels, spectra.shape[-1])
+
+
+# ---------------------------------------------------------------- PSF / LSF
+
+def gaussian_kernel_2d(m, n, sigma, dtype=np.float32):
+    """rubix/telescope/psf/kernels.py:5-31 (sigma in pixels)."""
+    x = np.arange(-((m - 1) / 2), ((m - 1) / 2) + 1).astype(dtype)
+    y = np.arange(-((n - 1) / 2), ((n - 1) / 2) + 1).astype(dtype)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    s = dtype(sigma)
+    v = np.exp(-(X ** 2 + Y ** 2) / (dtype(2) * s ** 2)).astype(dtype)
+    return (v / np.sum(v, dtype=dtype)).astype(dtype)
+
+
+def convolve2d_same(plane, kernel):
+    """Zero-padded true 2D convolution cropped to the input size, centred as
+    ``convolve2d(mode="same")`` (rubix/telescope/psf/psf.py:9-11): the crop
+    starts at (k-1)//2 of the full output on each axis."""
+    plane = np.asarray(plane)
+    kernel = np.asarray(kernel)
+    ny, nx = plane.shape[:2]
+    ky, kx = kernel.shape
+    sy, sx = (ky - 1) // 2, (kx - 1) // 2
+    out = np.zeros_like(plane)
+    for a in range(ky):
+        for b in range(kx):
+            dy = sy - a
+            dx = sx - b
+            # out[y, x] += K[a, b] * in[y + dy, x + dx]
+            y0, y1 = max(0, -dy), min(ny, ny - dy)
+            x0, x1 = max(0, -dx), min(nx, nx - dx)
+            if y0 >= y1 or x0 >= x1:
+                continue
+            out[y0:y1, x0:x1] += kernel[a, b] * plane[y0 + dy:y1 + dy, x0 + dx:x1 + dx]
+    return out
+
+
+def apply_psf(datacube, psf_kernel):
+    """rubix/telescope/psf/psf.py:33-58: convolve every lambda plane of the
+    (ny, nx, n_wave) cube with the kernel, mode "same".  The plane loop is
+    vectorised over lambda (the cube's trailing axis)."""
+    return convolve2d_same(np.asarray(datacube), np.asarray(psf_kernel))
+
+
+def gaussian1d(x, sigma):
+    """rubix/telescope/lsf/lsf.py:12-15."""
+    x = np.asarray(x)
+    r = np.exp(-0.5 * (x ** 2) / x.dtype.type(sigma) ** 2)
+    return r / np.sum(r, dtype=x.dtype)
+
+
+def lsf_kernel(sigma, wave_res, factor=12, dtype=np.float32):
+    """rubix/telescope/lsf/lsf.py:22-26: x = arange(-f*dl, f*dl + dl, dl)
+    (25 taps for factor 12 when the float arange is exact), Gaussian,
+    normalised to sum 1.  sigma in Angstrom."""
+    x = np.arange(-factor * wave_res, factor * wave_res + wave_res, wave_res).astype(dtype)
+    return gaussian1d(x, sigma).astype(dtype)
+
+
+def apply_lsf(datacube, lsf_sigma, wave_resolution, extend_factor=12, dtype=None):
+    """rubix/telescope/lsf/lsf.py:29-105: flatten to (n_spec, n_wave), full
+    convolution with the LSF kernel along lambda, crop
+    [factor : n + len(k) - 1 - factor], reshape back."""
+    cube = np.asarray(datacube)
+    dt = dtype or cube.dtype.type
+    k = lsf_kernel(lsf_sigma, wave_resolution, extend_factor, dtype=dt)
+    return apply_lsf_kernel(cube, k, extend_factor)
+
+
+def apply_lsf_kernel(datacube, k, extend_factor=12):
+    cube = np.asarray(datacube)
+    shape = cube.shape
+    spec = cube.reshape(-1, shape[-1])
+    n = spec.shape[1]
+    L = len(k)
+    full = np.zeros((spec.shape[0], n + L - 1), dtype=spec.dtype)
+    for t in range(L):
+        full[:, t:t + n] += k[t] * spec
+    end = n + L - 1 - extend_factor
+    return full[:, extend_factor:end].reshape(shape)
+
+
+# ---------------------------------------------------------------- path
+
+def plain_forward(stars, grid, geom, dtype=np.float32, chunk=4096,
+                  return_intermediates=False):
+    """The calc_ifu step chain (rubix/config/pipeline_config.yml:1-60) from
+    spaxel_assignment to convolve_lsf, at D = 1:
+
+    spaxel_assignment -> calculate_spectra -> scale_spectrum_by_mass ->
+    doppler_shift_and_resampling -> calculate_datacube -> convolve_psf ->
+    convolve_lsf.
+
+    ``stars``: dict coords (N,3), velocity (N,3), mass, age, metallicity.
+    ``grid``: dict ax_met, ax_age, wave, flux, log_axes.
+    ``geom``: dict edges, wave_target, z, psf_size, psf_sigma (pixels),
+    lsf_sigma (Angstrom), wave_res; the kernels are built in ``dtype``.
+    Stars are processed in chunks; the cube accumulates across chunks.
+    """
+    edges = np.asarray(geom["edges"], dtype=np.float32)
+    nb = len(edges) - 1
+    ids = square_spaxel_assignment(np.asarray(stars["coords"], dtype=np.float32), edges)
+    wave_t = np.asarray(geom["wave_target"], dtype=dtype)
+    ssp_wave = cosmological_doppler_shift(geom["z"], grid["wave"], dtype=dtype)
+    n = len(ids)
+    cube = np.zeros((nb * nb, len(wave_t)), dtype=dtype)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        spec = lookup_linear(stars["metallicity"][s:e], stars["age"][s:e],
+                             grid["ax_met"], grid["ax_age"], grid["flux"],
+                             log_axes=grid.get("log_axes", False), dtype=dtype)
+        spec = scale_spectrum_by_mass(spec, np.asarray(stars["mass"][s:e], dtype=dtype))
+        w = velocity_doppler_shift(ssp_wave, stars["velocity"][s:e], "z", dtype=dtype)
+        res = resample_spectra(spec, w, wave_t)
+        part = calculate_cube(res, ids[s:e], nb).reshape(nb * nb, -1)
+        cube += part
+    cube = cube.reshape(nb, nb, -1)
+    m = geom["psf_size"]
+    K = gaussian_kernel_2d(m, m, geom["psf_sigma"], dtype=dtype)
+    c_psf = apply_psf(cube, K)
+    c_lsf = apply_lsf_kernel(c_psf, lsf_kernel(geom["lsf_sigma"], geom["wave_res"], dtype=dtype))
+    if return_intermediates:
+        return dict(ids=ids, cube=cube, cube_psf=c_psf, cube_lsf=c_lsf)
+    return c_lsf

@@ -1,0 +1,288 @@
+"""Tensor-level wrappers of the HIP kernels (device tensors in, device tensors out).
+
+Every function here runs a hand-written HIP kernel through the C ABI on
+torch's current stream.  There is no CPU or torch fallback: inputs must be
+ROCm device tensors and the library must be built, otherwise the call raises.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import RbxSsp, RbxStars, RbxTarget, call, ptr, stream_handle
+
+SPEED_OF_LIGHT = 299792.458  # km/s, rubix/config/rubix_config.yml:8
+VELOCITY_COMPONENT = {"x": 0, "y": 1, "z": 2}
+
+
+def _need_gpu(*tensors):
+    for t in tensors:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise RuntimeError(
+                "rubix_amd kernels need ROCm device tensors (got "
+                f"{type(t).__name__} on {getattr(t, 'device', '?')}); there is no CPU path")
+
+
+def _f32(t, name):
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, got {t.dtype}")
+    return t.contiguous()
+
+
+# ---------------------------------------------------------------- spaxels
+
+def spaxel_assign(coords: torch.Tensor, edges: torch.Tensor) -> torch.Tensor:
+    """square_spaxel_assignment (rubix/telescope/utils.py:77-152) on the GPU.
+    coords (N, >=2) float32, edges (n_edges,) float32 -> int32 ids (N,)."""
+    _need_gpu(coords, edges)
+    coords = _f32(coords, "coords")
+    edges = _f32(edges, "edges")
+    if coords.dim() != 2 or coords.shape[1] < 2:
+        raise ValueError(f"coords must be (N, >=2), got {tuple(coords.shape)}")
+    n = coords.shape[0]
+    out = torch.empty(n, dtype=torch.int32, device=coords.device)
+    call("rbx_spaxel_assign", ptr(coords), n, coords.shape[1], ptr(edges), edges.numel(),
+         ptr(out), stream_handle(coords.device))
+    return out
+
+
+def spaxel_sort(ids: torch.Tensor, n_keys: int):
+    """Stable sort-by-spaxel index: (perm, sorted_ids, offsets[n_keys + 1])."""
+    _need_gpu(ids)
+    if ids.dtype != torch.int32:
+        raise TypeError("ids must be int32")
+    ids = ids.contiguous()
+    n = ids.numel()
+    dev = ids.device
+    perm = torch.empty(n, dtype=torch.int32, device=dev)
+    srt = torch.empty(n, dtype=torch.int32, device=dev)
+    offsets = torch.empty(n_keys + 1, dtype=torch.int32, device=dev)
+    L = _lib.load()
+    wsb = L.rbx_spaxel_sort_workspace_size(n, n_keys)
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+    call("rbx_spaxel_sort", ptr(ids), n, n_keys, ptr(perm), ptr(srt), ptr(offsets), ptr(ws), wsb,
+         stream_handle(dev))
+    return perm, srt, offsets
+
+
+# ---------------------------------------------------------------- SSP tables
+
+class PreparedSSP:
+    """Device-resident SSP grid for one redshift, with the tables the fused
+    kernels read: the (1+z)-shifted wavelengths (float32, as
+    rubix/core/ifu.py:261 computes them), per-template prefix sums of
+    flux * diff(wave_obs) (float64) and the knot-locator table."""
+
+    MAX_LOCATE = 1 << 23
+
+    def __init__(self, flux, ax_met, ax_age, wavelength, z, log_axes, device):
+        dev = torch.device(device)
+        self.device = dev
+        self.flux = torch.as_tensor(flux, dtype=torch.float32).to(dev).contiguous()
+        self.ax_met = torch.as_tensor(ax_met, dtype=torch.float32).to(dev).contiguous()
+        self.ax_age = torch.as_tensor(ax_age, dtype=torch.float32).to(dev).contiguous()
+        wave = torch.as_tensor(wavelength, dtype=torch.float32).to(dev)
+        _need_gpu(self.flux)
+        nm, na, nw = self.flux.shape
+        if self.ax_met.numel() != nm or self.ax_age.numel() != na or wave.numel() != nw:
+            raise ValueError("flux must be (n_met, n_age, n_wave) matching the axes")
+        if nm < 2 or na < 2 or nw < 2:
+            raise ValueError("the SSP grid needs >= 2 metallicities, ages and wavelengths")
+        self.z = float(z)
+        self.log_axes = bool(log_axes)
+        # (1 + z) * wavelength in float32 (python-float factor, array dtype)
+        self.wave_obs = (torch.tensor(1.0 + self.z, dtype=torch.float32, device=dev) * wave).contiguous()
+        w = self.wave_obs.double().cpu().numpy()
+        dw = np.diff(w)
+        if not np.all(dw > 0):
+            raise ValueError("SSP wavelengths must be strictly increasing")
+        for ax, nm_ in ((self.ax_met, "metallicity"), (self.ax_age, "age")):
+            if not bool(torch.all(ax[1:] > ax[:-1])):
+                raise ValueError(f"SSP {nm_} axis must be strictly increasing")
+        h = float(dw.min())
+        span = float(w[-1] - w[0])
+        nloc = int(math.floor(span / h)) + 2
+        if nloc > self.MAX_LOCATE:
+            nloc = self.MAX_LOCATE
+            h = span / (nloc - 2)
+        self.n_locate = nloc
+        self.locate_x0 = float(np.float32(w[0]))
+        self.locate_inv_h = float(np.float32(1.0 / h))
+        self.cum = torch.empty(nm * na * nw, dtype=torch.float64, device=dev)
+        self.locate = torch.empty(nloc, dtype=torch.int32, device=dev)
+        self.shape = (nm, na, nw)
+        self._struct = self._make_struct()
+        call("rbx_ssp_prepare", self._struct, ptr(self.cum), ptr(self.locate), stream_handle(dev))
+
+    def _make_struct(self):
+        nm, na, nw = self.shape
+        s = RbxSsp()
+        s.flux = self.flux.data_ptr()
+        s.ax_met = self.ax_met.data_ptr()
+        s.ax_age = self.ax_age.data_ptr()
+        s.wave_obs = self.wave_obs.data_ptr()
+        s.cum = self.cum.data_ptr()
+        s.locate = self.locate.data_ptr()
+        s.n_met, s.n_age, s.n_wave = nm, na, nw
+        s.n_locate = self.n_locate
+        s.locate_x0 = self.locate_x0
+        s.locate_inv_h = self.locate_inv_h
+        s.log_axes = int(self.log_axes)
+        return s
+
+    @property
+    def struct(self):
+        return self._struct
+
+
+class TargetGrid:
+    """Telescope wavelength axis on the device (rubix/core/ifu.py:254-255)
+    with calculate_diff (rubix/spectra/ifu.py:83-102) and its min/max."""
+
+    def __init__(self, wave, device):
+        w = torch.as_tensor(wave, dtype=torch.float32).to(device).contiguous()
+        _need_gpu(w)
+        if w.numel() < 1:
+            raise ValueError("empty target wavelength grid")
+        self.wave = w
+        d = torch.zeros_like(w)
+        if w.numel() > 1:
+            d[1:] = w[1:] - w[:-1]
+        self.dwave = d.contiguous()
+        self.wmin = float(w.min())
+        self.wmax = float(w.max())
+        s = RbxTarget()
+        s.wave = w.data_ptr()
+        s.dwave = self.dwave.data_ptr()
+        s.n = w.numel()
+        s.wmin = self.wmin
+        s.wmax = self.wmax
+        self._struct = s
+
+    @property
+    def n(self):
+        return self.wave.numel()
+
+    @property
+    def struct(self):
+        return self._struct
+
+
+# ---------------------------------------------------------------- fused cube
+
+@dataclass
+class CubeContext:
+    """What the fused forward leaves for the adjoint (workspace + inputs)."""
+    stars: RbxStars
+    workspace: torch.Tensor
+    keep: tuple
+
+
+def _stars_struct(mass, age, met, velocity, ids, n_spaxel, n_keys, vel_comp, c):
+    n = mass.numel()
+    for t, nm in ((age, "age"), (met, "metallicity"), (ids, "ids")):
+        if t.numel() != n:
+            raise ValueError(f"{nm} has {t.numel()} entries, mass has {n}")
+    if velocity.dim() != 2 or velocity.shape[0] != n:
+        raise ValueError(f"velocity must be (N, 3), got {tuple(velocity.shape)}")
+    s = RbxStars()
+    s.n = n
+    s.mass = mass.data_ptr()
+    s.age = age.data_ptr()
+    s.metallicity = met.data_ptr()
+    s.velocity = velocity.data_ptr()
+    s.vel_stride = velocity.shape[1]
+    s.vel_comp = vel_comp
+    s.n_spaxel = n_spaxel
+    s.n_keys = n_keys
+    s.ids = ids.data_ptr()
+    s.speed_of_light = c
+    return s
+
+
+def cube_forward(ssp: PreparedSSP, target: TargetGrid, mass, age, met, velocity, ids,
+                 n_spaxel: int, n_keys: int, vel_comp: int = 2, c: float = SPEED_OF_LIGHT):
+    """Fused calculate_spectra -> scale_spectrum_by_mass ->
+    doppler_shift_and_resampling -> calculate_datacube.  Returns the raw cube
+    (n_spaxel, n_target) and the context the adjoint needs."""
+    _need_gpu(mass, age, met, velocity, ids)
+    mass = _f32(mass, "mass")
+    age = _f32(age, "age")
+    met = _f32(met, "metallicity")
+    velocity = _f32(velocity, "velocity")
+    if ids.dtype != torch.int32:
+        raise TypeError("ids must be int32")
+    ids = ids.contiguous()
+    st = _stars_struct(mass, age, met, velocity, ids, n_spaxel, n_keys, vel_comp, c)
+    L = _lib.load()
+    wsb = L.rbx_cube_workspace_size(st.n, n_keys, target.n)
+    dev = mass.device
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    cube = torch.empty(n_spaxel, target.n, dtype=torch.float32, device=dev)
+    call("rbx_cube_fwd", ssp.struct, target.struct, st, ptr(cube), ptr(ws), wsb, stream_handle(dev))
+    ctx = CubeContext(st, ws, (mass, age, met, velocity, ids))
+    return cube, ctx
+
+
+def cube_backward(ssp: PreparedSSP, target: TargetGrid, ctx: CubeContext, dcube,
+                  need=(True, True, True, True)):
+    """Adjoint of cube_forward: (dmass, dage, dmet, dv_los) for dL/dcube."""
+    _need_gpu(dcube)
+    dcube = _f32(dcube, "dcube")
+    n = ctx.stars.n
+    dev = dcube.device
+    outs = [torch.empty(n, dtype=torch.float32, device=dev) if k else None for k in need]
+    call("rbx_cube_bwd", ssp.struct, target.struct, ctx.stars, ptr(dcube), *[ptr(o) for o in outs],
+         ptr(ctx.workspace), ctx.workspace.numel(), stream_handle(dev))
+    return tuple(outs)
+
+
+# ---------------------------------------------------------------- convolutions
+
+def conv_axis(x, taps, n_outer, n_axis, n_inner, start):
+    """1D convolution along the middle axis of an (n_outer, n_axis, n_inner)
+    view: out[o,x,i] = sum_t taps[t] * in[o, x + start - t, i] (zero pad)."""
+    _need_gpu(x, taps)
+    x = _f32(x, "input")
+    taps = _f32(taps, "taps")
+    out = torch.empty_like(x)
+    call("rbx_conv_axis", ptr(x), ptr(out), n_outer, n_axis, n_inner, ptr(taps), taps.numel(),
+         start, stream_handle(x.device))
+    return out
+
+
+def conv2d_planes(cube, kernel, sy=None, sx=None):
+    """Per-lambda-plane 2D convolution, mode "same" (general kernel)."""
+    _need_gpu(cube, kernel)
+    cube = _f32(cube, "cube")
+    kernel = _f32(kernel, "kernel")
+    ny, nx, nl = cube.shape
+    ky, kx = kernel.shape
+    sy = (ky - 1) // 2 if sy is None else sy
+    sx = (kx - 1) // 2 if sx is None else sx
+    out = torch.empty_like(cube)
+    call("rbx_conv2d_planes", ptr(cube), ptr(out), ny, nx, nl, ptr(kernel), ky, kx, sy, sx,
+         stream_handle(cube.device))
+    return out
+
+
+def conv_axis_dot(a, b, taps, n_outer, n_axis, n_inner, start):
+    """sum(a * conv_axis(b)) as a device float64 scalar (fixed order)."""
+    _need_gpu(a, b, taps)
+    a = _f32(a, "a")
+    b = _f32(b, "b")
+    taps = _f32(taps, "taps")
+    L = _lib.load()
+    total = n_outer * n_axis * n_inner
+    wsb = L.rbx_dot_workspace_size(total)
+    ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=a.device)
+    res = torch.empty((), dtype=torch.float64, device=a.device)
+    call("rbx_conv_axis_dot", ptr(a), ptr(b), n_outer, n_axis, n_inner, ptr(taps), taps.numel(),
+         start, ptr(res), ptr(ws), wsb, stream_handle(a.device))
+    return res

@@ -1,0 +1,89 @@
+"""GPU adjoints against the gradient oracle (torch float64 autograd through the
+restated reference op graph), for the chi^2 loss of the differentiation
+workload: chi2 = sum((C(theta) - C_tgt)^2 / s^2), C_tgt rendered from a second
+seed with Gaussian noise at S/N = 30.
+
+Tolerance: norm-wise relative error <= 1e-5 per gradient tensor (mass, age,
+metallicity, v_los, sigma_psf, sigma_lsf) against float64.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_numpy as R, ref_torch as RT
+from rubix_amd import ops, synthetic as S
+from rubix_amd.autograd import render
+from tests.helpers import rel_err, with_kernels
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+TOL = 1e-5
+
+
+def _target(w, seed=2):
+    st2 = S.make_stars(w.n_stars, seed=seed)
+    clean = R.plain_forward(st2, w.grid, w.geom, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    noise_sd = np.abs(clean) / 30.0
+    tgt = clean + rng.normal(0.0, 1.0, clean.shape) * noise_sd
+    sig = np.maximum(noise_sd, 1e-6 * np.abs(clean).max())
+    return tgt, sig
+
+
+def _oracle_grads(w, tgt, sig):
+    st = w.stars
+    f64 = torch.float64
+    mass = torch.tensor(st["mass"], dtype=f64, requires_grad=True)
+    age = torch.tensor(st["age"], dtype=f64, requires_grad=True)
+    met = torch.tensor(st["metallicity"], dtype=f64, requires_grad=True)
+    vl = torch.tensor(st["velocity"][:, 2], dtype=f64, requires_grad=True)
+    sp = torch.tensor(w.geom["psf_sigma"], dtype=f64, requires_grad=True)
+    sl = torch.tensor(w.geom["lsf_sigma"], dtype=f64, requires_grad=True)
+    ids = torch.tensor(R.square_spaxel_assignment(st["coords"], w.geom["edges"]).astype(np.int64))
+    _, _, c2 = RT.full_forward(mass, age, met, vl, ids, sp, sl, w.grid, w.geom, f64)
+    loss = torch.sum((c2 - torch.tensor(tgt)) ** 2 / torch.tensor(sig) ** 2)
+    loss.backward()
+    return float(loss), [x.grad.numpy() for x in (mass, age, met, vl, sp, sl)]
+
+
+def _gpu_grads(w, tgt, sig):
+    st = {k: torch.as_tensor(v, device=DEV) for k, v in w.stars.items()}
+    for k in ("mass", "age", "metallicity", "velocity"):
+        st[k].requires_grad_(True)
+    g = w.grid
+    ssp = ops.PreparedSSP(g["flux"], g["ax_met"], g["ax_age"], g["wave"], w.geom["z"],
+                          g["log_axes"], DEV)
+    tg = ops.TargetGrid(w.geom["wave_target"], DEV)
+    edges = torch.as_tensor(w.geom["edges"], device=DEV)
+    sp = torch.tensor(w.geom["psf_sigma"], dtype=torch.float32, device=DEV, requires_grad=True)
+    sl = torch.tensor(w.geom["lsf_sigma"], dtype=torch.float32, device=DEV, requires_grad=True)
+    c2 = render(st["coords"], st["velocity"], st["mass"], st["age"], st["metallicity"], edges, ssp,
+                tg, w.geom["nb"], sp, w.geom["psf_size"], sl, w.geom["wave_res"])
+    tt = torch.as_tensor(tgt, dtype=torch.float32, device=DEV)
+    ss = torch.as_tensor(sig, dtype=torch.float32, device=DEV)
+    loss = torch.sum((c2 - tt) ** 2 / ss ** 2)
+    loss.backward()
+    vel = st["velocity"].grad.cpu().numpy()
+    assert np.all(vel[:, :2] == 0)
+    return float(loss), [st["mass"].grad.cpu().numpy(), st["age"].grad.cpu().numpy(),
+                         st["metallicity"].grad.cpu().numpy(), vel[:, 2],
+                         sp.grad.cpu().numpy(), sl.grad.cpu().numpy()]
+
+
+@pytest.mark.parametrize("cfg", ["cfg1", "cfg2"])
+def test_gradients_match_oracle(cfg):
+    if cfg == "cfg1":
+        w = with_kernels(S.config1(n_stars=600))
+    else:
+        w = with_kernels(S.config2(n_stars=150))
+    tgt, sig = _target(w)
+    lo, go = _oracle_grads(w, tgt, sig)
+    lg, gg = _gpu_grads(w, tgt, sig)
+    names = ["mass", "age", "metallicity", "v_los", "sigma_psf", "sigma_lsf"]
+    print(f"{cfg} loss gpu={lg:.6e} oracle={lo:.6e} rel={abs(lg - lo) / abs(lo):.3e}")
+    errs = {}
+    for nm, a, b in zip(names, gg, go):
+        errs[nm] = rel_err(a, b)
+        print(f"{cfg} d/d{nm}: rel={errs[nm]:.3e} |g|={np.linalg.norm(b):.3e}")
+    for nm, e in errs.items():
+        assert e <= TOL, (nm, e)
