@@ -145,6 +145,55 @@ int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target,
                  float* dage, float* dmet, float* dvlos, void* workspace,
                  size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------- per step
+ * One kernel per rubix.core step, materialising the same per-star arrays the
+ * reference chain stores (the "plain" path; HBM-bound by construction).
+ *
+ * calculate_spectra (rubix/core/ifu.py:63-121 with the interp2d linear lookup
+ * of rubix/spectra/ssp/grid.py:113-120): spectra (n, n_wave) of star i at
+ * (metallicity[i], age[i]); 0 outside the grid.  Needs flux and axes only. */
+int rbx_ssp_lookup(const rbx_ssp* ssp, const float* met, const float* age, int64_t n,
+                   float* spectra, void* stream);
+/* Adjoint: dmet[i], dage[i] from dspectra (n, n_wave); 0 outside the grid. */
+int rbx_ssp_lookup_bwd(const rbx_ssp* ssp, const float* met, const float* age, int64_t n,
+                       const float* dspectra, float* dmet, float* dage, void* stream);
+
+/* scale_spectrum_by_mass (rubix/core/ifu.py:149-157): out[i, k] = in[i, k] *
+ * scale[i] (out may alias in).  Its adjoint is the same call with the
+ * incoming gradient, plus rbx_row_dot for d/dscale. */
+int rbx_scale_rows(const float* in, const float* scale, int64_t n, int32_t n_wave, float* out,
+                   void* stream);
+/* out[i] = sum_k a[i, k] * b[i, k] (fixed-order reduction). */
+int rbx_row_dot(const float* a, const float* b, int64_t n, int32_t n_wave, float* out,
+                void* stream);
+
+/* doppler_shift_and_resampling (rubix/core/ifu.py:269-293 with
+ * velocity_doppler_shift rubix/spectra/ifu.py:190 and resample_spectrum
+ * rubix/spectra/ifu.py:223-266): spectra (n, ssp->n_wave) on the knots
+ * wave_obs * exp(v / c) -> out (n, target->n).  Uses ssp->wave_obs and the
+ * locator table only. */
+int rbx_resample(const rbx_ssp* ssp, const rbx_target* target, const float* spectra,
+                 const float* velocity, int64_t vel_stride, int32_t vel_comp,
+                 float speed_of_light, int64_t n, float* out, void* stream);
+/* Adjoint: dspectra (n, ssp->n_wave) and dvlos (n) from dout (n, target->n).
+ * Target wavelengths must be increasing; 10*n_target + 2*n_wave bytes of LDS
+ * per workgroup (<= 64 KiB). */
+int rbx_resample_bwd(const rbx_ssp* ssp, const rbx_target* target, const float* spectra,
+                     const float* velocity, int64_t vel_stride, int32_t vel_comp,
+                     float speed_of_light, int64_t n, const float* dout, float* dspectra,
+                     float* dvlos, void* stream);
+
+/* calculate_datacube (rubix/spectra/ifu.py:269-288, segment_sum into
+ * n_spaxel segments, ids outside [0, n_spaxel) dropped): deterministic
+ * sort-by-spaxel + chunked fixed-order sums.  cube (n_spaxel, n_wave). */
+size_t rbx_segment_cube_workspace_size(int64_t n, int32_t n_keys, int32_t n_wave);
+int rbx_segment_cube(const float* spectra, const int32_t* ids, int64_t n, int32_t n_spaxel,
+                     int32_t n_keys, int32_t n_wave, float* cube, void* workspace,
+                     size_t workspace_bytes, void* stream);
+/* Adjoint (gather): dspectra[i] = dcube[ids[i]] (0 for dropped ids). */
+int rbx_segment_cube_bwd(const float* dcube, const int32_t* ids, int64_t n, int32_t n_spaxel,
+                         int32_t n_wave, float* dspectra, void* stream);
+
 /* ---------------------------------------------------------------- PSF / LSF
  * 1D convolution along the middle axis of an (n_outer, n_axis, n_inner)
  * array with zero padding and a crop start:
@@ -172,12 +221,13 @@ int rbx_conv2d_planes(const float* in, float* out, int32_t ny, int32_t nx,
                       int32_t nl, const float* kernel, int32_t ky, int32_t kx,
                       int32_t sy, int32_t sx, void* stream);
 
-/* dot = sum(a * conv_axis(b)) in double with a fixed reduction order (used
- * for dL/dsigma of the PSF and LSF).  partial: workspace of
- * rbx_dot_workspace_size(total) bytes; result: one device double. */
+/* dot = sum(a * conv_axis(b)) with double taps, a double convolution and a
+ * fixed-order double reduction (used for dL/dsigma of the PSF and LSF, whose
+ * derivative taps sum to zero).  workspace: rbx_dot_workspace_size(total)
+ * bytes; result: one device double. */
 size_t rbx_dot_workspace_size(int64_t total);
 int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer,
-                      int32_t n_axis, int64_t n_inner, const float* taps,
+                      int32_t n_axis, int64_t n_inner, const double* taps,
                       int32_t n_taps, int32_t start, double* result,
                       void* workspace, size_t workspace_bytes, void* stream);
 

@@ -172,7 +172,9 @@ def velocity_doppler_shift(wavelength, velocity, direction="z",
         raise ValueError(
             f"{direction} is not a valid direction. Supported directions are 'x', 'y', or 'z'.")
     v = np.asarray(velocity, dtype=dtype)[:, comp[direction]]
-    a = np.exp(v / dtype(c)).astype(dtype)
+    # v / c in the array's float type; exp correctly rounded to that type
+    # (evaluated in float64 and rounded once)
+    a = np.exp((v / dtype(c)).astype(np.float64)).astype(dtype)
     return (np.asarray(wavelength, dtype=dtype)[None, :] * a[:, None]).astype(dtype)
 
 
@@ -336,7 +338,7 @@ def apply_lsf_kernel(datacube, k, extend_factor=12):
 # ---------------------------------------------------------------- path
 
 def plain_forward(stars, grid, geom, dtype=np.float32, chunk=4096,
-                  return_intermediates=False):
+                  return_intermediates=False, ref_knots=True):
     """The calc_ifu step chain (rubix/config/pipeline_config.yml:1-60) from
     spaxel_assignment to convolve_lsf, at D = 1:
 
@@ -349,12 +351,20 @@ def plain_forward(stars, grid, geom, dtype=np.float32, chunk=4096,
     ``geom``: dict edges, wave_target, z, psf_size, psf_sigma (pixels),
     lsf_sigma (Angstrom), wave_res; the kernels are built in ``dtype``.
     Stars are processed in chunks; the cube accumulates across chunks.
+
+    ``ref_knots`` (float64 only): the shifted source knots are the float32
+    values the reference computes, fl32(fl32((1+z) wave) * fl32(exp(v/c))),
+    and everything after them is float64.  The knots are inputs of the
+    interpolation whose rounding moves interpolation weights by ~1e-4, so this
+    is the float64 evaluation of the reference's own float32 algorithm;
+    ``ref_knots=False`` gives exact arithmetic throughout.
     """
     edges = np.asarray(geom["edges"], dtype=np.float32)
     nb = len(edges) - 1
     ids = square_spaxel_assignment(np.asarray(stars["coords"], dtype=np.float32), edges)
     wave_t = np.asarray(geom["wave_target"], dtype=dtype)
-    ssp_wave = cosmological_doppler_shift(geom["z"], grid["wave"], dtype=dtype)
+    kd = np.float32 if (ref_knots or dtype == np.float32) else dtype
+    ssp_wave = cosmological_doppler_shift(geom["z"], grid["wave"], dtype=kd)
     n = len(ids)
     cube = np.zeros((nb * nb, len(wave_t)), dtype=dtype)
     for s in range(0, n, chunk):
@@ -363,7 +373,7 @@ def plain_forward(stars, grid, geom, dtype=np.float32, chunk=4096,
                              grid["ax_met"], grid["ax_age"], grid["flux"],
                              log_axes=grid.get("log_axes", False), dtype=dtype)
         spec = scale_spectrum_by_mass(spec, np.asarray(stars["mass"][s:e], dtype=dtype))
-        w = velocity_doppler_shift(ssp_wave, stars["velocity"][s:e], "z", dtype=dtype)
+        w = velocity_doppler_shift(ssp_wave, stars["velocity"][s:e], "z", dtype=kd).astype(dtype)
         res = resample_spectra(spec, w, wave_t)
         part = calculate_cube(res, ids[s:e], nb).reshape(nb * nb, -1)
         cube += part

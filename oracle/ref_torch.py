@@ -149,11 +149,25 @@ def apply_lsf(cube, k, factor=12):
     return full[:, 0, factor:end].reshape(shape)
 
 
-def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype):
+def reference_knots(ssp_wave32, vel_los, a):
+    """Shifted knots with the reference's float32 values and the derivative of
+    the exact product: value fl32(fl32((1+z) wave) * fl32(exp(v/c))), gradient
+    d(wave * a)/da (straight-through the rounding, as autodiff of the float32
+    reference treats it)."""
+    import numpy as np
+    v32 = vel_los.detach().to(torch.float32)
+    a32 = torch.exp((v32 / np.float32(SPEED_OF_LIGHT)).double()).to(torch.float32)
+    w32 = (ssp_wave32[None, :] * a32[:, None]).to(a.dtype)
+    w = ssp_wave32.to(a.dtype)[None, :] * a[:, None]
+    return w + (w32 - w).detach()
+
+
+def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots=True):
     """Raw datacube (before PSF/LSF) from per-star tensors, the
     calculate_spectra -> scale_spectrum_by_mass ->
     doppler_shift_and_resampling -> calculate_datacube chain
-    (rubix/core/ifu.py:63-121,149-157,269-293,327-339)."""
+    (rubix/core/ifu.py:63-121,149-157,269-293,327-339).  ``ref_knots``: see
+    reference_knots (float64 evaluation of the reference's float32 knots)."""
     axm = torch.as_tensor(grid["ax_met"], dtype=dtype)
     axa = torch.as_tensor(grid["ax_age"], dtype=dtype)
     flux = torch.as_tensor(grid["flux"], dtype=dtype)
@@ -165,12 +179,14 @@ def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype):
     spec = lookup_linear(one_m, one_a, axm, axa, flux, log_axes)
     spec = torch.where(outside[:, None], torch.zeros_like(spec), spec)
     spec = spec * mass[:, None]
-    wave = torch.as_tensor(grid["wave"], dtype=dtype)
-    ssp_wave = (1 + geom["z"]) * wave
-    if dtype == torch.float32:
-        ssp_wave = torch.tensor(float(1 + geom["z"]), dtype=dtype) * wave
+    wave32 = torch.as_tensor(grid["wave"], dtype=torch.float32)
+    ssp_wave32 = torch.tensor(float(1 + geom["z"]), dtype=torch.float32) * wave32
     a = torch.exp(vel_los / SPEED_OF_LIGHT)
-    w = ssp_wave[None, :] * a[:, None]
+    if ref_knots or dtype == torch.float32:
+        w = reference_knots(ssp_wave32, vel_los, a)
+    else:
+        ssp_wave = (1 + geom["z"]) * torch.as_tensor(grid["wave"], dtype=dtype)
+        w = ssp_wave[None, :] * a[:, None]
     tw = torch.as_tensor(geom["wave_target"], dtype=dtype)
     res = resample(spec, w, tw)
     nb = len(geom["edges"]) - 1
@@ -178,10 +194,10 @@ def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype):
 
 
 def full_forward(mass, age, met, vel_los, ids, sigma_psf, sigma_lsf, grid, geom,
-                 dtype=torch.float64):
+                 dtype=torch.float64, ref_knots=True):
     """Whole path with tensors for every differentiable input; returns
     (raw cube, cube after PSF, cube after LSF)."""
-    cube = star_cube(mass, age, met, vel_los, ids, grid, geom, dtype)
+    cube = star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots)
     m = geom["psf_size"]
     K = gaussian_kernel_2d(m, m, sigma_psf)
     c1 = apply_psf(cube, K)

@@ -53,6 +53,20 @@ SIGNATURES = {
                              c_void_p, c_size_t, c_void_p]),
     "rbx_cube_bwd": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars), c_void_p,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "rbx_ssp_lookup": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "rbx_ssp_lookup_bwd": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                   c_void_p, c_void_p]),
+    "rbx_scale_rows": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "rbx_row_dot": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "rbx_resample": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), c_void_p, c_void_p, c_int64,
+                             c_int32, c_float, c_int64, c_void_p, c_void_p]),
+    "rbx_resample_bwd": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), c_void_p, c_void_p, c_int64,
+                                 c_int32, c_float, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rbx_segment_cube_workspace_size": (c_size_t, [c_int64, c_int32, c_int32]),
+    "rbx_segment_cube": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int32, c_int32, c_void_p,
+                                 c_void_p, c_size_t, c_void_p]),
+    "rbx_segment_cube_bwd": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p,
+                                     c_void_p]),
     "rbx_conv_axis": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_void_p, c_int32,
                               c_int32, c_void_p]),
     "rbx_psf_sep": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p,

@@ -54,6 +54,10 @@ def _dev(a, device):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float32, device=device)
 
 
+def _dev64(a, device):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
+
+
 # ---------------------------------------------------------------- cube
 
 class FusedDatacube(torch.autograd.Function):
@@ -92,7 +96,7 @@ class GaussianPSF(torch.autograd.Function):
         ny, nx, nl = cube.shape
         g, dg = gaussian_taps_2d_factor(size, float(sigma))
         dev = cube.device
-        gt, dgt = _dev(g, dev), _dev(dg, dev)
+        gt, dgt = _dev(g, dev), _dev64(dg, dev)
         start = (size - 1) // 2
         cube = cube.contiguous()
         tmp = ops.conv_axis(cube, gt, ny, nx, nl, start)             # along x
@@ -135,7 +139,7 @@ class GaussianLSF(torch.autograd.Function):
         ns = cube.numel() // nl
         k, dk = lsf_taps(float(sigma), float(wave_res), factor)
         dev = cube.device
-        kt, dkt = _dev(k, dev), _dev(dk, dev)
+        kt, dkt = _dev(k, dev), _dev64(dk, dev)
         cube = cube.contiguous()
         out = ops.conv_axis(cube, kt, ns, nl, 1, factor)
         ctx.save_for_backward(cube)
@@ -159,6 +163,97 @@ class GaussianLSF(torch.autograd.Function):
         if ctx.needs_input_grad[1]:
             dsig = ops.conv_axis_dot(gout, cube, dkt, ns, nl, 1, ctx.factor).to(torch.float32)
         return gin, dsig, None, None
+
+
+# ---------------------------------------------------------------- per step
+
+class SSPLookup(torch.autograd.Function):
+    """calculate_spectra: (metallicity, age) -> spectra (N, n_wave)."""
+
+    @staticmethod
+    def forward(ctx, met, age, ssp):
+        met_d, age_d = met.detach().contiguous(), age.detach().contiguous()
+        ctx.save_for_backward(met_d, age_d)
+        ctx.ssp = ssp
+        return ops.ssp_lookup(ssp, met_d, age_d)
+
+    @staticmethod
+    def backward(ctx, dspec):
+        met, age = ctx.saved_tensors
+        dm, da = ops.ssp_lookup_bwd(ctx.ssp, met, age, dspec.contiguous())
+        return dm.view_as(met), da.view_as(age), None
+
+
+class ScaleByMass(torch.autograd.Function):
+    """scale_spectrum_by_mass: spectra[i, :] * mass[i]."""
+
+    @staticmethod
+    def forward(ctx, spectra, mass):
+        spectra_d, mass_d = spectra.detach().contiguous(), mass.detach().contiguous()
+        ctx.save_for_backward(spectra_d, mass_d)
+        return ops.scale_rows(spectra_d, mass_d)
+
+    @staticmethod
+    def backward(ctx, dout):
+        spectra, mass = ctx.saved_tensors
+        dout = dout.contiguous()
+        ds = ops.scale_rows(dout, mass) if ctx.needs_input_grad[0] else None
+        dm = ops.row_dot(dout, spectra).view_as(mass) if ctx.needs_input_grad[1] else None
+        return ds, dm
+
+
+class Resample(torch.autograd.Function):
+    """doppler_shift_and_resampling: spectra (N, n_wave) + velocity (N, 3) ->
+    (N, n_target)."""
+
+    @staticmethod
+    def forward(ctx, spectra, velocity, ssp, target, vel_comp, c):
+        spectra_d, vel_d = spectra.detach().contiguous(), velocity.detach().contiguous()
+        ctx.save_for_backward(spectra_d, vel_d)
+        ctx.args = (ssp, target, vel_comp, c)
+        return ops.resample(ssp, target, spectra_d, vel_d, vel_comp, c)
+
+    @staticmethod
+    def backward(ctx, dout):
+        spectra, vel = ctx.saved_tensors
+        ssp, target, vel_comp, c = ctx.args
+        need_s, need_v = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        ds, dv = ops.resample_bwd(ssp, target, spectra, vel, dout.contiguous(), vel_comp, c,
+                                  need_spectra=need_s, need_velocity=need_v)
+        dvel = None
+        if need_v:
+            dvel = torch.zeros_like(vel)
+            dvel[:, vel_comp] = dv
+        return ds, dvel, None, None, None, None
+
+
+class SegmentCube(torch.autograd.Function):
+    """calculate_datacube: segment sum of (N, n_target) rows by spaxel id."""
+
+    @staticmethod
+    def forward(ctx, spectra, ids, n_spaxel, n_keys):
+        ctx.save_for_backward(ids)
+        ctx.n_spaxel = n_spaxel
+        return ops.segment_cube(spectra.detach(), ids, n_spaxel, n_keys)
+
+    @staticmethod
+    def backward(ctx, dcube):
+        (ids,) = ctx.saved_tensors
+        return ops.segment_cube_bwd(dcube.contiguous(), ids, ctx.n_spaxel), None, None, None
+
+
+def render_plain(coords, velocity, mass, age, met, edges, ssp, target, nb, psf_sigma, psf_size,
+                 lsf_sigma, wave_res, vel_comp=2, c=ops.SPEED_OF_LIGHT):
+    """The plain per-step chain (one kernel per reference step, spectra
+    materialised), differentiable like render()."""
+    ids = ops.spaxel_assign(coords, edges)
+    nbins = edges.numel() - 1
+    spec = SSPLookup.apply(met, age, ssp)
+    spec = ScaleByMass.apply(spec, mass)
+    spec = Resample.apply(spec, velocity, ssp, target, vel_comp, c)
+    raw = SegmentCube.apply(spec, ids, nb * nb, nbins * nbins).view(nb, nb, target.n)
+    c1 = GaussianPSF.apply(raw, psf_sigma, psf_size)
+    return GaussianLSF.apply(c1, lsf_sigma, wave_res)
 
 
 # ---------------------------------------------------------------- composition

@@ -62,9 +62,11 @@ constexpr int kDotThreads = 256;
 constexpr int kDotBlocksMax = 4096;
 
 // Per-block partial of sum(a * conv_axis(b)) in double, fixed tree order.
+// The sigma-derivative taps sum to zero, so conv(dtaps, b) cancels strongly:
+// taps and the convolution are double.
 __global__ void __launch_bounds__(kDotThreads) conv_dot_partial_kernel(
     const float* __restrict__ a, const float* __restrict__ b, int64_t n_outer, int n_axis,
-    int64_t n_inner, const float* __restrict__ taps, int n_taps, int start,
+    int64_t n_inner, const double* __restrict__ taps, int n_taps, int start,
     double* __restrict__ partial) {
   __shared__ double sm[kDotThreads];
   const int64_t total = n_outer * n_axis * n_inner;
@@ -75,12 +77,12 @@ __global__ void __launch_bounds__(kDotThreads) conv_dot_partial_kernel(
     const int x = int(ox % n_axis);
     const int64_t o = ox / n_axis;
     const float* src = b + o * n_axis * n_inner + i;
-    float c = 0.f;
+    double c = 0.0;
     for (int t = 0; t < n_taps; ++t) {
       const int xs = x + start - t;
-      if (xs >= 0 && xs < n_axis) c += taps[t] * src[int64_t(xs) * n_inner];
+      if (xs >= 0 && xs < n_axis) c += taps[t] * double(src[int64_t(xs) * n_inner]);
     }
-    acc += double(a[idx]) * double(c);
+    acc += double(a[idx]) * c;
   }
   sm[threadIdx.x] = acc;
   __syncthreads();
@@ -168,7 +170,7 @@ size_t rbx_dot_workspace_size(int64_t total) {
 }
 
 int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer, int32_t n_axis,
-                      int64_t n_inner, const float* taps, int32_t n_taps, int32_t start,
+                      int64_t n_inner, const double* taps, int32_t n_taps, int32_t start,
                       double* result, void* workspace, size_t workspace_bytes, void* stream) {
   using namespace rbx;
   if (!a || !b || !taps || !result || n_outer < 0 || n_axis < 1 || n_inner < 1 || n_taps < 1)

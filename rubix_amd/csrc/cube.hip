@@ -36,19 +36,12 @@
 #include <math.h>
 
 #include "rbx_common.h"
+#include "ssp_device.h"
 
 namespace rbx {
 
-int spaxel_sort_impl(const int32_t* ids, int64_t n, int32_t n_keys, int32_t* perm,
-                     int32_t* sorted, int32_t* offsets, void* ws, size_t ws_bytes,
-                     hipStream_t s);
-size_t sort_workspace_size(int64_t n, int32_t n_keys);
-int exclusive_scan_i32(int32_t* data, int64_t n, int32_t* blocks, hipStream_t s);
-void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 
-constexpr int kChunk = 256;   // stars per reduction chunk
 constexpr int kTile = 64;     // target bins per wavefront in cube_accum
-constexpr float kLn10 = 2.302585092994046f;
 
 struct __align__(16) StarState {
   int32_t row;      // flux row of corner (iz-1, ja-1); -1 = contributes nothing
@@ -65,114 +58,6 @@ static_assert(sizeof(StarState) == 64, "StarState must be 64 bytes");
 static_assert(sizeof(rbx_ssp) == 80, "rbx_ssp layout");
 static_assert(sizeof(rbx_target) == 32, "rbx_target layout");
 static_assert(sizeof(rbx_stars) == 80, "rbx_stars layout");
-
-struct ChunkDesc {
-  int32_t spaxel, begin, end, pad;
-};
-
-struct SspView {
-  const float* __restrict__ flux;
-  const float* __restrict__ ax_met;
-  const float* __restrict__ ax_age;
-  const float* __restrict__ wave;
-  const double* __restrict__ cum;
-  const int32_t* __restrict__ loc;
-  int n_met, n_age, n_wave, n_loc;
-  float loc_x0, loc_inv_h;
-  int log_axes;
-};
-
-static SspView view_of(const rbx_ssp& s) {
-  SspView v;
-  v.flux = s.flux; v.ax_met = s.ax_met; v.ax_age = s.ax_age; v.wave = s.wave_obs;
-  v.cum = s.cum; v.loc = s.locate; v.n_met = s.n_met; v.n_age = s.n_age;
-  v.n_wave = s.n_wave; v.n_loc = s.n_locate; v.loc_x0 = s.locate_x0;
-  v.loc_inv_h = s.locate_inv_h; v.log_axes = s.log_axes;
-  return v;
-}
-
-// Shifted knot, rounded like the reference's w = fl(wave * a) (no FMA).
-__device__ inline float knot(const SspView& g, int k, float a) {
-  return __fmul_rn(g.wave[k], a);
-}
-
-// Number of knots with fl(wave[k] * a) <= x, minus one (in [-1, n-1]):
-// searchsorted(w, x, 'right') - 1 with the exact float predicate.  The
-// locator table gives a start within a knot or two; the walks terminate
-// after at most n steps for any input.
-__device__ inline int locate(const SspView& g, float x, float a, float inv_a) {
-  const float fm = (x * inv_a - g.loc_x0) * g.loc_inv_h;
-  int m = 0;
-  if (fm > 0.f) m = fm < float(g.n_loc - 1) ? int(fm) : g.n_loc - 1;
-  int k = g.loc[m];
-  const int n = g.n_wave;
-  while (k + 1 < n && knot(g, k + 1, a) <= x) ++k;
-  while (k >= 0 && knot(g, k, a) > x) --k;
-  return k;
-}
-
-// jnp.interp bookkeeping for one target x: segment [i-1, i] with
-// i = clip(k+1, 1, n-1), theta = (x - w[i-1]) / (w[i] - w[i-1]), and the
-// clamp side (-1: x < w[0] -> S[0]; +1: x > w[n-1] -> S[n-1]; 0 interior).
-struct Seg {
-  int i;
-  int side;
-  float theta;
-  float wl, wr;
-};
-
-__device__ inline Seg segment(const SspView& g, float x, float a, float inv_a) {
-  const int k = locate(g, x, a, inv_a);
-  const int n = g.n_wave;
-  Seg s;
-  s.i = k + 1 < 1 ? 1 : (k + 1 > n - 1 ? n - 1 : k + 1);
-  s.wl = knot(g, s.i - 1, a);
-  s.wr = knot(g, s.i, a);
-  s.side = (k < 0) ? -1 : ((k >= n - 1 && x > s.wr) ? 1 : 0);
-  s.theta = (x - s.wl) / (s.wr - s.wl);
-  return s;
-}
-
-// Bilinear cell of one star (interp2d linear): corner weights, their
-// derivatives w.r.t. the (possibly log10) query coordinates, and the chain
-// factors to the stored metallicity / age.
-struct Cell {
-  int row;
-  bool inside;
-  float w[4], gu[4], gs[4];
-  float du_dZ, ds_dt;
-};
-
-__device__ inline Cell make_cell(const SspView& g, float Z, float t) {
-  Cell c;
-  const float u = g.log_axes ? log10f(Z) : Z;
-  const float s = g.log_axes ? log10f(t) : t;
-  const int nm = g.n_met, na = g.n_age;
-  c.inside = (u >= g.ax_met[0]) && (u <= g.ax_met[nm - 1]) && (s >= g.ax_age[0]) &&
-             (s <= g.ax_age[na - 1]);
-  int iz = upper_bound_f(g.ax_met, nm, u);
-  int ja = upper_bound_f(g.ax_age, na, s);
-  iz = iz < 1 ? 1 : (iz > nm - 1 ? nm - 1 : iz);
-  ja = ja < 1 ? 1 : (ja > na - 1 ? na - 1 : ja);
-  const float x1 = g.ax_met[iz - 1], x2 = g.ax_met[iz];
-  const float y1 = g.ax_age[ja - 1], y2 = g.ax_age[ja];
-  const float ax2 = x2 - u, ax1 = u - x1, ay2 = y2 - s, ay1 = s - y1;
-  const float inv = 1.0f / ((x2 - x1) * (y2 - y1));
-  c.w[0] = ax2 * ay2 * inv;
-  c.w[1] = ax1 * ay2 * inv;
-  c.w[2] = ax2 * ay1 * inv;
-  c.w[3] = ax1 * ay1 * inv;
-  c.gu[0] = -ay2 * inv; c.gu[1] = ay2 * inv; c.gu[2] = -ay1 * inv; c.gu[3] = ay1 * inv;
-  c.gs[0] = -ax2 * inv; c.gs[1] = -ax1 * inv; c.gs[2] = ax2 * inv; c.gs[3] = ax1 * inv;
-  c.row = (iz - 1) * na + (ja - 1);
-  c.du_dZ = g.log_axes ? 1.0f / (Z * kLn10) : 1.0f;
-  c.ds_dt = g.log_axes ? 1.0f / (t * kLn10) : 1.0f;
-  return c;
-}
-
-__device__ inline int corner_row(int row, int c, int n_age) {
-  return row + ((c & 1) ? n_age : 0) + ((c & 2) ? 1 : 0);
-}
 
 // sum_{k=kl..kh} f_c[k] (wave[k] - wave[k-1]) for each corner, in double.
 __device__ inline void window_sums(const SspView& g, int row, int kl, int kh, double out[4]) {
@@ -202,7 +87,7 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
     s.pad = 0;
     const float m = st.mass[i];
     const float v = st.velocity[int64_t(i) * st.vel_stride + st.vel_comp];
-    const float a = expf(v / st.speed_of_light);
+    const float a = doppler_factor(v, st.speed_of_light);
     s.a = a;
     s.inv_a = 1.0f / a;
     s.m = m;
@@ -310,7 +195,7 @@ __global__ void chunk_fill_kernel(const int32_t* __restrict__ offsets, int32_t n
 __global__ void __launch_bounds__(64) cube_accum_kernel(
     SspView g, rbx_target tg, const StarState* __restrict__ state,
     const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr,
-    int n_tiles, int tiles_per_xcd, float* __restrict__ partial) {
+    int n_tiles, int tiles_per_xcd, double* __restrict__ partial) {
   const int b = blockIdx.x;
   const int xcd = b & 7;
   const int q = b >> 3;
@@ -321,17 +206,20 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
   const int j = tile * kTile + threadIdx.x;
   const bool jv = j < tg.n;
   const float x = tg.wave[jv ? j : tg.n - 1];
-  float acc = 0.f;
+  // double accumulator: the cube is a sum of up to chunk-size positive
+  // terms per lane; accumulating in double keeps the forward's rounding at
+  // the level of the individual float32 terms.
+  double acc = 0.0;
   for (int r = d.begin; r < d.end; ++r) {
     const StarState s = state[r];
     if (s.row < 0) continue;
     const Seg sg = segment(g, x, s.a, s.inv_a);
-    acc += interp_value(g, s, sg) * s.sf;
+    acc += double(interp_value(g, s, sg) * s.sf);
   }
   if (jv) partial[int64_t(chunk) * tg.n + j] = acc;
 }
 
-__global__ void cube_combine_kernel(const float* __restrict__ partial,
+__global__ void cube_combine_kernel(const double* __restrict__ partial,
                                     const int32_t* __restrict__ chunk_off, int32_t n_spaxel,
                                     int32_t nt, float* __restrict__ cube) {
   const int64_t total = int64_t(n_spaxel) * nt;
@@ -339,9 +227,9 @@ __global__ void cube_combine_kernel(const float* __restrict__ partial,
     const int32_t p = int32_t(idx / nt);
     const int32_t j = int32_t(idx - int64_t(p) * nt);
     const int32_t c0 = chunk_off[p], c1 = chunk_off[p + 1];
-    float s = 0.f;
+    double s = 0.0;
     for (int32_t c = c0; c < c1; ++c) s += partial[int64_t(c) * nt + j];
-    cube[idx] = s;
+    cube[idx] = float(s);
   }
 }
 
@@ -369,76 +257,81 @@ __global__ void __launch_bounds__(256) star_bwd_kernel(
   const float* fr[4];
   for (int c = 0; c < 4; ++c) fr[c] = g.flux + int64_t(corner_row(s.row, c, g.n_age)) * nw;
   const float* grow = dcube + int64_t(s.spaxel) * tg.n;
-  float Vg[4] = {0, 0, 0, 0}, Ag[4] = {0, 0, 0, 0}, Vd[4] = {0, 0, 0, 0}, Ad[4] = {0, 0, 0, 0};
+  // The Doppler sums (A*) cancel strongly: sum_j g_j dP_j/da is a small
+  // difference of terms ~ slope * lambda / dlambda, and dsf/da is itself a
+  // near-cancellation of T/a/Nn and T/Nn^2 dNn/da.  In float32 these lose
+  // ~3e-4 of the v_los gradient (a float32 autograd of the reference graph
+  // shows the same loss), so they are accumulated in double; the value sums
+  // (V*) do not cancel and stay float32.
+  float Vg[4] = {0, 0, 0, 0}, Vd[4] = {0, 0, 0, 0};
+  double Ag[4] = {0, 0, 0, 0}, Ad[4] = {0, 0, 0, 0};
   for (int j = lane; j < tg.n; j += 64) {
     const float x = tg.wave[j];
     const float gj = grow[j];
     const float dj = tg.dwave[j];
     const Seg sg = segment(g, x, s.a, s.inv_a);
     const int i0 = sg.i - 1, i1 = sg.i;
-    // dtheta/da scaled: dR/da = -(f1 - f0) * kappa for interior targets
+    // dR/da = -(f1 - f0) * kappa for interior targets (0 when clamped)
     const float kappa = (g.wave[i0] + sg.theta * (g.wave[i1] - g.wave[i0])) / (sg.wr - sg.wl);
+    const double gk = sg.side == 0 ? -double(gj) * double(kappa) : 0.0;
+    const double dk = sg.side == 0 ? -double(dj) * double(kappa) : 0.0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const float f0 = fr[c][i0], f1 = fr[c][i1];
       const float df = f1 - f0;
       float R = f0 + sg.theta * df;
-      float D = -df * kappa;
-      if (sg.side < 0) { R = f0; D = 0.f; }
-      if (sg.side > 0) { R = f1; D = 0.f; }
+      if (sg.side < 0) R = f0;
+      if (sg.side > 0) R = f1;
       Vg[c] += gj * R;
-      Ag[c] += gj * D;
       Vd[c] += dj * R;
-      Ad[c] += dj * D;
+      Ag[c] += gk * double(df);
+      Ad[c] += dk * double(df);
     }
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     Vg[c] = wave_sum(Vg[c]);
-    Ag[c] = wave_sum(Ag[c]);
     Vd[c] = wave_sum(Vd[c]);
+    Ag[c] = wave_sum(Ag[c]);
     Ad[c] = wave_sum(Ad[c]);
   }
   if (lane != 0) return;
   const Cell cell = make_cell(g, st.metallicity[i], st.age[i]);
   double win[4];
   window_sums(g, s.row, s.kl, s.kh, win);
-  const float m = s.m, a = s.a, sf = s.sf, T = s.T, Nn = s.Nn;
-  float Pg_m = 0.f, Pg_u = 0.f, Pg_s = 0.f, Pg_a = 0.f;
-  float Pd_u = 0.f, Pd_s = 0.f, Pd_a = 0.f;
-  double W_u = 0.0, W_s = 0.0;
+  const double m = s.m, a = s.a, sf = s.sf, T = s.T, Nn = s.Nn;
+  double Pg_m = 0, Pg_u = 0, Pg_s = 0, Pg_a = 0, Pd_u = 0, Pd_s = 0, Pd_a = 0, W_u = 0, W_s = 0;
   for (int c = 0; c < 4; ++c) {
-    Pg_m += s.w[c] * Vg[c];
-    Pg_u += cell.gu[c] * Vg[c];
-    Pg_s += cell.gs[c] * Vg[c];
-    Pg_a += s.w[c] * Ag[c];
-    Pd_u += cell.gu[c] * Vd[c];
-    Pd_s += cell.gs[c] * Vd[c];
-    Pd_a += s.w[c] * Ad[c];
-    W_u += double(cell.gu[c]) * win[c];
-    W_s += double(cell.gs[c]) * win[c];
+    const double w = s.w[c], gu = cell.gu[c], gs = cell.gs[c];
+    Pg_m += w * Vg[c];
+    Pg_u += gu * Vg[c];
+    Pg_s += gs * Vg[c];
+    Pg_a += w * Ag[c];
+    Pd_u += gu * Vd[c];
+    Pd_s += gs * Vd[c];
+    Pd_a += w * Ad[c];
+    W_u += gu * win[c];
+    W_s += gs * win[c];
   }
-  // sum_j g_j dP_j/dtheta = m * Pg_theta, dNn/dtheta = m * Pd_theta,
-  // dT/du = m a W_u, dT/ds = m a W_s, dT/da = T / a.
-  const float sbar = m * Pg_m;  // sum_j g_j P_j
-  const float raw = T / Nn;
-  const bool live = isfinite(raw);
-  float dsf_u = 0.f, dsf_s = 0.f, dsf_a = 0.f;
+  // sum_j g_j dP_j/dtheta = m Pg_theta, dNn/dtheta = m Pd_theta,
+  // dT/du = m a W_u, dT/ds = m a W_s, dT/da = T / a; sbar = sum_j g_j P_j.
+  const double sbar = m * Pg_m;
+  const bool live = isfinite(s.T / s.Nn);
+  double dsf_u = 0, dsf_s = 0, dsf_a = 0;
   if (live) {
-    const float inv_n = 1.0f / Nn;
-    const float t_n2 = T * inv_n * inv_n;
-    dsf_u = float(double(m) * double(a) * W_u) * inv_n - t_n2 * (m * Pd_u);
-    dsf_s = float(double(m) * double(a) * W_s) * inv_n - t_n2 * (m * Pd_s);
-    dsf_a = (T / a) * inv_n - t_n2 * (m * Pd_a);
+    const double t_n2 = T / (Nn * Nn);
+    dsf_u = m * a * W_u / Nn - t_n2 * (m * Pd_u);
+    dsf_s = m * a * W_s / Nn - t_n2 * (m * Pd_s);
+    dsf_a = (T / a) / Nn - t_n2 * (m * Pd_a);
   }
-  const float g_m = sf * Pg_m;
-  const float g_u = sf * (m * Pg_u) + sbar * dsf_u;
-  const float g_s = sf * (m * Pg_s) + sbar * dsf_s;
-  const float g_a = sf * (m * Pg_a) + sbar * dsf_a;
-  if (dmass) dmass[i] = g_m;
-  if (dmet) dmet[i] = g_u * cell.du_dZ;
-  if (dage) dage[i] = g_s * cell.ds_dt;
-  if (dvlos) dvlos[i] = g_a * a / st.speed_of_light;
+  const double g_m = sf * Pg_m;
+  const double g_u = sf * (m * Pg_u) + sbar * dsf_u;
+  const double g_s = sf * (m * Pg_s) + sbar * dsf_s;
+  const double g_a = sf * (m * Pg_a) + sbar * dsf_a;
+  if (dmass) dmass[i] = float(g_m);
+  if (dmet) dmet[i] = float(g_u * cell.du_dZ);
+  if (dage) dage[i] = float(g_s * cell.ds_dt);
+  if (dvlos) dvlos[i] = float(g_a * a / double(st.speed_of_light));
 }
 
 // ------------------------------------------------------------ ssp tables
@@ -488,7 +381,7 @@ struct CubeLayout {
   int32_t *perm, *skey, *offsets, *chunk_off, *scan_blocks;
   StarState* state;
   ChunkDesc* desc;
-  float* partial;
+  double* partial;
   void* sort_ws;
   size_t sort_ws_bytes;
   int64_t max_chunks;
@@ -505,7 +398,7 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, CubeLayout*
   auto* blocks = c.template take<int32_t>((size_t(n_keys) + 1) / 4096 + 2);
   auto* state = c.template take<StarState>(n > 0 ? n : 1);
   auto* desc = c.template take<ChunkDesc>(max_chunks > 0 ? max_chunks : 1);
-  auto* part = c.template take<float>(size_t(max_chunks > 0 ? max_chunks : 1) * nt);
+  auto* part = c.template take<double>(size_t(max_chunks > 0 ? max_chunks : 1) * nt);
   auto* sws = c.template take<char>(sort_bytes);
   if (L) {
     L->perm = reinterpret_cast<int32_t*>(perm);
@@ -515,7 +408,7 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, CubeLayout*
     L->scan_blocks = reinterpret_cast<int32_t*>(blocks);
     L->state = reinterpret_cast<StarState*>(state);
     L->desc = reinterpret_cast<ChunkDesc*>(desc);
-    L->partial = reinterpret_cast<float*>(part);
+    L->partial = reinterpret_cast<double*>(part);
     L->sort_ws = sws;
     L->sort_ws_bytes = sort_bytes;
     L->max_chunks = max_chunks;
@@ -555,6 +448,23 @@ static int validate(const rbx_ssp* ssp, const rbx_target* tg, const rbx_stars* s
     return fail(RBX_EINVAL, "rbx_cube: need 1 <= n_spaxel <= n_keys");
   if (!(st->speed_of_light > 0.f)) return fail(RBX_EINVAL, "rbx_cube: speed_of_light must be > 0");
   return RBX_OK;
+}
+
+int build_chunks(const int32_t* offsets, int32_t ns, int32_t* chunk_off, int32_t* scan_blocks,
+                 ChunkDesc* desc, hipStream_t s) {
+  hipLaunchKernelGGL(chunk_count_kernel, dim3(grid_for(int64_t(ns) + 1, 256)), dim3(256), 0, s,
+                     offsets, ns, chunk_off);
+  int rc = exclusive_scan_i32(chunk_off, int64_t(ns) + 1, scan_blocks, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3(grid_for(ns, 256)), dim3(256), 0, s, offsets, ns,
+                     chunk_off, desc);
+  return check_launch("build_chunks");
+}
+
+void combine_chunks(const double* partial, const int32_t* chunk_off, int32_t ns, int32_t nt,
+                    float* cube, hipStream_t s) {
+  hipLaunchKernelGGL(cube_combine_kernel, dim3(grid_for(int64_t(ns) * nt, 256)), dim3(256), 0, s,
+                     partial, chunk_off, ns, nt, cube);
 }
 
 }  // namespace rbx
@@ -608,12 +518,8 @@ int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* 
                        L.state);
   }
   const int32_t ns = stars->n_spaxel;
-  hipLaunchKernelGGL(chunk_count_kernel, dim3(grid_for(int64_t(ns) + 1, 256)), dim3(256), 0, s,
-                     L.offsets, ns, L.chunk_off);
-  rc = exclusive_scan_i32(L.chunk_off, int64_t(ns) + 1, L.scan_blocks, s);
+  rc = build_chunks(L.offsets, ns, L.chunk_off, L.scan_blocks, L.desc, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(chunk_fill_kernel, dim3(grid_for(ns, 256)), dim3(256), 0, s, L.offsets, ns,
-                     L.chunk_off, L.desc);
   const int n_tiles = (nt + kTile - 1) / kTile;
   const int tpx = (n_tiles + 7) / 8;
   const int64_t max_chunks = (n + kChunk - 1) / kChunk + ns;
@@ -623,8 +529,7 @@ int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* 
     hipLaunchKernelGGL(cube_accum_kernel, dim3(blocks), dim3(64), 0, s, g, *target, L.state,
                        L.desc, L.chunk_off + ns, n_tiles, tpx, L.partial);
   }
-  hipLaunchKernelGGL(cube_combine_kernel, dim3(grid_for(int64_t(ns) * nt, 256)), dim3(256), 0, s,
-                     L.partial, L.chunk_off, ns, nt, cube);
+  combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s);
   return check_launch("rbx_cube_fwd");
 }
 

@@ -157,6 +157,7 @@ class TargetGrid:
         self.dwave = d.contiguous()
         self.wmin = float(w.min())
         self.wmax = float(w.max())
+        self.ascending = bool(torch.all(w[1:] > w[:-1])) if w.numel() > 1 else True
         s = RbxTarget()
         s.wave = w.data_ptr()
         s.dwave = self.dwave.data_ptr()
@@ -273,11 +274,14 @@ def conv2d_planes(cube, kernel, sy=None, sx=None):
 
 
 def conv_axis_dot(a, b, taps, n_outer, n_axis, n_inner, start):
-    """sum(a * conv_axis(b)) as a device float64 scalar (fixed order)."""
+    """sum(a * conv_axis(b)) as a device float64 scalar (fixed order); taps
+    float64."""
     _need_gpu(a, b, taps)
     a = _f32(a, "a")
     b = _f32(b, "b")
-    taps = _f32(taps, "taps")
+    if taps.dtype != torch.float64:
+        raise TypeError("conv_axis_dot taps must be float64")
+    taps = taps.contiguous()
     L = _lib.load()
     total = n_outer * n_axis * n_inner
     wsb = L.rbx_dot_workspace_size(total)
@@ -286,3 +290,116 @@ def conv_axis_dot(a, b, taps, n_outer, n_axis, n_inner, start):
     call("rbx_conv_axis_dot", ptr(a), ptr(b), n_outer, n_axis, n_inner, ptr(taps), taps.numel(),
          start, ptr(res), ptr(ws), wsb, stream_handle(a.device))
     return res
+
+
+# ---------------------------------------------------------------- per step
+
+def _rows(x, name):
+    x = _f32(x, name)
+    if x.dim() != 2:
+        raise ValueError(f"{name} must be 2-D (stars, wavelengths), got {tuple(x.shape)}")
+    return x
+
+
+def ssp_lookup(ssp: PreparedSSP, met, age):
+    """calculate_spectra: interp2d linear lookup -> (N, n_wave)."""
+    _need_gpu(met, age)
+    met = _f32(met, "metallicity").reshape(-1)
+    age = _f32(age, "age").reshape(-1)
+    n = met.numel()
+    out = torch.empty(n, ssp.shape[2], dtype=torch.float32, device=met.device)
+    call("rbx_ssp_lookup", ssp.struct, ptr(met), ptr(age), n, ptr(out), stream_handle(met.device))
+    return out
+
+
+def ssp_lookup_bwd(ssp: PreparedSSP, met, age, dspec):
+    _need_gpu(met, age, dspec)
+    met = _f32(met, "metallicity").reshape(-1)
+    age = _f32(age, "age").reshape(-1)
+    dspec = _rows(dspec, "dspectra")
+    n = met.numel()
+    dm = torch.empty(n, dtype=torch.float32, device=met.device)
+    da = torch.empty(n, dtype=torch.float32, device=met.device)
+    call("rbx_ssp_lookup_bwd", ssp.struct, ptr(met), ptr(age), n, ptr(dspec), ptr(dm), ptr(da),
+         stream_handle(met.device))
+    return dm, da
+
+
+def scale_rows(x, scale):
+    """scale_spectrum_by_mass: x[i, :] * scale[i]."""
+    _need_gpu(x, scale)
+    x = _rows(x, "spectra")
+    scale = _f32(scale, "mass").reshape(-1)
+    out = torch.empty_like(x)
+    call("rbx_scale_rows", ptr(x), ptr(scale), x.shape[0], x.shape[1], ptr(out),
+         stream_handle(x.device))
+    return out
+
+
+def row_dot(a, b):
+    _need_gpu(a, b)
+    a = _rows(a, "a")
+    b = _rows(b, "b")
+    out = torch.empty(a.shape[0], dtype=torch.float32, device=a.device)
+    call("rbx_row_dot", ptr(a), ptr(b), a.shape[0], a.shape[1], ptr(out), stream_handle(a.device))
+    return out
+
+
+def resample(ssp: PreparedSSP, target: TargetGrid, spectra, velocity, vel_comp=2,
+             c=SPEED_OF_LIGHT):
+    """doppler_shift_and_resampling on materialised spectra (N, n_wave)."""
+    _need_gpu(spectra, velocity)
+    spectra = _rows(spectra, "spectra")
+    velocity = _f32(velocity, "velocity")
+    n = spectra.shape[0]
+    if spectra.shape[1] != ssp.shape[2] or velocity.shape[0] != n:
+        raise ValueError("spectra must be (N, n_wave) and velocity (N, 3)")
+    out = torch.empty(n, target.n, dtype=torch.float32, device=spectra.device)
+    call("rbx_resample", ssp.struct, target.struct, ptr(spectra), ptr(velocity), velocity.shape[1],
+         vel_comp, c, n, ptr(out), stream_handle(spectra.device))
+    return out
+
+
+def resample_bwd(ssp: PreparedSSP, target: TargetGrid, spectra, velocity, dout, vel_comp=2,
+                 c=SPEED_OF_LIGHT, need_spectra=True, need_velocity=True):
+    _need_gpu(spectra, velocity, dout)
+    if not target.ascending:
+        raise ValueError("the resampling adjoint needs increasing target wavelengths")
+    spectra = _rows(spectra, "spectra")
+    velocity = _f32(velocity, "velocity")
+    dout = _rows(dout, "dout")
+    n = spectra.shape[0]
+    dev = spectra.device
+    ds = torch.empty_like(spectra) if need_spectra else None
+    dv = torch.empty(n, dtype=torch.float32, device=dev) if need_velocity else None
+    call("rbx_resample_bwd", ssp.struct, target.struct, ptr(spectra), ptr(velocity),
+         velocity.shape[1], vel_comp, c, n, ptr(dout), ptr(ds), ptr(dv), stream_handle(dev))
+    return ds, dv
+
+
+def segment_cube(spectra, ids, n_spaxel, n_keys):
+    """calculate_datacube: deterministic segment sum -> (n_spaxel, n_wave)."""
+    _need_gpu(spectra, ids)
+    spectra = _rows(spectra, "spectra")
+    if ids.dtype != torch.int32:
+        raise TypeError("ids must be int32")
+    ids = ids.contiguous().reshape(-1)
+    n, nw = spectra.shape
+    L = _lib.load()
+    wsb = L.rbx_segment_cube_workspace_size(n, n_keys, nw)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=spectra.device)
+    cube = torch.empty(n_spaxel, nw, dtype=torch.float32, device=spectra.device)
+    call("rbx_segment_cube", ptr(spectra), ptr(ids), n, n_spaxel, n_keys, nw, ptr(cube), ptr(ws),
+         wsb, stream_handle(spectra.device))
+    return cube
+
+
+def segment_cube_bwd(dcube, ids, n_spaxel):
+    _need_gpu(dcube, ids)
+    dcube = _rows(dcube, "dcube")
+    ids = ids.contiguous().reshape(-1)
+    n = ids.numel()
+    out = torch.empty(n, dcube.shape[1], dtype=torch.float32, device=dcube.device)
+    call("rbx_segment_cube_bwd", ptr(dcube), ptr(ids), n, n_spaxel, dcube.shape[1], ptr(out),
+         stream_handle(dcube.device))
+    return out

@@ -46,7 +46,7 @@ def cfg2_sample(n=2_000, seed=0):
 
 
 @functools.lru_cache(maxsize=None)
-def oracle(name, dtype_name, n):
+def oracle(name, dtype_name, n, ref_knots=True):
     w = cfg1(n) if name == "cfg1" else cfg2_sample(n)
     return R.plain_forward(w.stars, w.grid, w.geom, dtype=np.dtype(dtype_name).type,
-                           return_intermediates=True)
+                           return_intermediates=True, ref_knots=ref_knots)

@@ -48,6 +48,7 @@ def test_fused_cube_matches_oracle(name, n):
     ids, raw, c1, c2 = _render(w)
     o64 = oracle(name, "float64", n)
     o32 = oracle(name, "float32", n)
+    ox = oracle(name, "float64", n, ref_knots=False)
     np.testing.assert_array_equal(ids, o64["ids"])
     report = {}
     for key, got in (("cube", raw), ("cube_psf", c1), ("cube_lsf", c2)):
@@ -56,6 +57,7 @@ def test_fused_cube_matches_oracle(name, n):
         m64 = max_rel(got, o64[key])
         report[key] = (e64, e32, m64)
         print(f"{name} {key}: rel64={e64:.3e} rel32={e32:.3e} maxrel64={m64:.3e} "
+              f"exact-arithmetic rel={rel_err(got, ox[key]):.3e} "
               f"(oracle32 vs 64: {rel_err(o32[key], o64[key]):.3e})")
     for key, (e64, e32, m64) in report.items():
         assert e64 <= TOL, (key, e64)
