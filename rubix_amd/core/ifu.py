@@ -1,0 +1,161 @@
+"""IFU steps (rubix/core/ifu.py:28-341) on the GPU.
+
+Plain per-step path, each step writing the same field the reference step
+writes (shapes after reshape_data, leading device axis 1):
+  calculate_spectra             stars.spectra (1, N, n_wave)
+  scale_spectrum_by_mass        stars.spectra *= mass
+  doppler_shift_and_resampling  stars.spectra (1, N, n_target)
+  calculate_datacube            stars.datacube (nb, nb, n_target)
+and the fused step calculate_datacube_fused, which produces the same
+stars.datacube from (mass, age, metallicity, velocity, pixel_assignment)
+without materialising per-star spectra.  Every step is differentiable
+(hand-written adjoints, rubix_amd.autograd).
+"""
+from __future__ import annotations
+
+from typing import Callable
+
+import torch
+
+from .. import ops
+from ..autograd import FusedDatacube, Resample, ScaleByMass, SegmentCube, SSPLookup
+from ..config import config as pkg_config
+from ..logger import get_logger
+from .data import RubixData
+from .ssp import get_ssp, lookup_method
+from .telescope import get_telescope, step_device
+
+
+def _speed_of_light():
+    return float(pkg_config["constants"]["SPEED_OF_LIGHT"])
+
+
+def _los_component():
+    d = pkg_config["ifu"]["doppler"]["velocity_direction"]
+    if d not in ops.VELOCITY_COMPONENT:
+        raise ValueError(
+            f"{d} is not a valid direction. Supported directions are 'x', 'y', or 'z'.")
+    return ops.VELOCITY_COMPONENT[d]
+
+
+def _dev(t, dev):
+    return t.to(dev, torch.float32) if t.device != dev or t.dtype != torch.float32 else t
+
+
+def get_calculate_spectra(config: dict) -> Callable:
+    """rubix/core/ifu.py:28-123 (lookup on the device-0 row)."""
+    logger = get_logger(config.get("logger", None))
+    ssp = get_ssp(config)
+    lookup_method(config)
+    dev = step_device(config)
+    prepared = ssp.prepared(0.0, dev)
+
+    def calculate_spectra(rubixdata: RubixData) -> RubixData:
+        logger.info("Calculating IFU cube...")
+        met = torch.atleast_1d(rubixdata.stars.metallicity)
+        age = torch.atleast_1d(rubixdata.stars.age)
+        spec = SSPLookup.apply(_dev(met[0], dev), _dev(age[0], dev), prepared)
+        rubixdata.stars.spectra = spec.unsqueeze(0)
+        return rubixdata
+
+    return calculate_spectra
+
+
+def get_scale_spectrum_by_mass(config: dict) -> Callable:
+    """rubix/core/ifu.py:126-159."""
+    logger = get_logger(config.get("logger", None))
+    dev = step_device(config)
+
+    def scale_spectrum_by_mass(rubixdata: RubixData) -> RubixData:
+        logger.info("Scaling Spectra by Mass...")
+        s = rubixdata.stars.spectra
+        m = rubixdata.stars.mass
+        out = ScaleByMass.apply(s.reshape(-1, s.shape[-1]), _dev(m, dev).reshape(-1))
+        rubixdata.stars.spectra = out.reshape(s.shape)
+        return rubixdata
+
+    return scale_spectrum_by_mass
+
+
+def get_doppler_shift_and_resampling(config: dict) -> Callable:
+    """rubix/core/ifu.py:225-295: knots (1+z) wave exp(v_los / c) and the
+    flux-conserving resampling onto the telescope wavelengths."""
+    logger = get_logger(config.get("logger", None))
+    comp = _los_component()
+    z = config["galaxy"]["dist_z"]
+    tel = get_telescope(config)
+    ssp = get_ssp(config)
+    dev = step_device(config)
+    prepared = ssp.prepared(z, dev)
+    target = ops.TargetGrid(tel.wave_seq, dev)
+    c = _speed_of_light()
+
+    def process_particle(particle):
+        if particle.spectra is None:
+            return None
+        logger.info("Doppler shifting and resampling spectra...")
+        s = particle.spectra
+        v = _dev(particle.velocity, dev)
+        out = Resample.apply(s.reshape(-1, s.shape[-1]), v.reshape(-1, v.shape[-1]), prepared,
+                             target, comp, c)
+        return out.reshape(*s.shape[:-1], target.n)
+
+    def doppler_shift_and_resampling(rubixdata: RubixData) -> RubixData:
+        for name in ("stars", "gas"):
+            part = getattr(rubixdata, name)
+            part.spectra = process_particle(part)
+        return rubixdata
+
+    return doppler_shift_and_resampling
+
+
+def get_calculate_datacube(config: dict) -> Callable:
+    """rubix/core/ifu.py:298-341: segment sum by spaxel (ids >= nb^2 dropped)
+    with a deterministic, atomic-free reduction."""
+    logger = get_logger(config.get("logger", None))
+    nb = int(get_telescope(config).sbin)
+    dev = step_device(config)
+
+    def calculate_datacube(rubixdata: RubixData) -> RubixData:
+        logger.info("Calculating Data Cube...")
+        s = rubixdata.stars.spectra
+        ids = rubixdata.stars.pixel_assignment.to(dev).reshape(-1).to(torch.int32)
+        cube = SegmentCube.apply(s.reshape(-1, s.shape[-1]), ids, nb * nb, nb * nb)
+        rubixdata.stars.datacube = cube.reshape(nb, nb, s.shape[-1])
+        return rubixdata
+
+    return calculate_datacube
+
+
+def get_calculate_datacube_fused(config: dict) -> Callable:
+    """Fused replacement of calculate_spectra -> scale_spectrum_by_mass ->
+    doppler_shift_and_resampling -> calculate_datacube: writes
+    stars.datacube (nb, nb, n_target), leaves stars.spectra None."""
+    logger = get_logger(config.get("logger", None))
+    lookup_method(config)
+    comp = _los_component()
+    z = config["galaxy"]["dist_z"]
+    tel = get_telescope(config)
+    nb = int(tel.sbin)
+    ssp = get_ssp(config)
+    dev = step_device(config)
+    prepared = ssp.prepared(z, dev)
+    target = ops.TargetGrid(tel.wave_seq, dev)
+    c = _speed_of_light()
+
+    def calculate_datacube_fused(rubixdata: RubixData) -> RubixData:
+        logger.info("Calculating Data Cube (fused)...")
+        st = rubixdata.stars
+        mass = _dev(st.mass, dev).reshape(-1)
+        age = _dev(st.age, dev).reshape(-1)
+        met = _dev(st.metallicity, dev).reshape(-1)
+        vel = _dev(st.velocity, dev)
+        vel = vel.reshape(-1, vel.shape[-1])
+        ids = st.pixel_assignment.to(dev).reshape(-1).to(torch.int32)
+        cube = FusedDatacube.apply(mass, age, met, vel, ids, prepared, target, nb * nb, nb * nb,
+                                   comp, c)
+        st.datacube = cube.reshape(nb, nb, target.n)
+        st.spectra = None
+        return rubixdata
+
+    return calculate_datacube_fused

@@ -192,3 +192,34 @@ def config4_sizes(n_gal=64, seed=3):
     ns = np.round(10 ** rng.uniform(math.log10(5e4), math.log10(2e6), n_gal)).astype(np.int64)
     nbs = rng.choice([25, 50, 100], n_gal)
     return ns, nbs
+
+
+SSP_TEMPLATE_NAMES = {(6, 20, 1000): "synthetic_6x20x1000", (12, 100, 5000): "synthetic_12x100x5000"}
+
+
+def step_config(w: Workload, device="cuda"):
+    """Step-API config for a workload: a telescope definition with the
+    workload's spaxel size and wavelength step, explicit spaxel edges and
+    target wavelengths, the Gaussian PSF (sigma in pixels) and LSF, and the
+    registered synthetic SSP template of the same grid."""
+    g = w.geom
+    nb = g["nb"]
+    arcsec = g["fov_arcsec"] / nb
+    nm, na, nw = w.grid["flux"].shape
+    name = SSP_TEMPLATE_NAMES[(nm, na, nw)]
+    wt = np.asarray(g["wave_target"], dtype=np.float64)
+    tel = {"fov": g["fov_arcsec"], "spatial_res": arcsec,
+           "wave_range": [float(wt[0]), float(wt[-1]) + g["wave_res"] / 2], "wave_res": g["wave_res"],
+           "lsf_fwhm": None, "signal_to_noise": None, "aperture_type": "square",
+           "pixel_type": "square"}
+    return {
+        "device": device,
+        "galaxy": {"dist_z": g["z"]},
+        "telescopes": {f"SYNTHETIC_{w.name}": tel},
+        "telescope": {"name": f"SYNTHETIC_{w.name}", "spatial_bin_edges": g["edges"],
+                      "wave_seq": g["wave_target"],
+                      "psf": {"name": "gaussian", "size": g["psf_size"], "sigma": g["psf_sigma"]},
+                      "lsf": {"sigma": g["lsf_sigma"]}},
+        "ssp": {"template": {"name": name}, "method": "linear"},
+        "cosmology": {"name": "PLANCK15"},
+    }
