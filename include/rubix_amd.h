@@ -90,12 +90,14 @@ int rbx_ssp_prepare(const rbx_ssp* ssp, double* cum_out, int32_t* locate_out,
 
 /* Telescope wavelength axis (rubix/core/ifu.py:254-255).  dwave is
  * calculate_diff(wave) (first element 0, rubix/spectra/ifu.py:83-102);
- * wmin/wmax are min/max of wave. */
+ * wmin/wmax are min/max of wave; ascending != 0 when wave is non-decreasing
+ * (enables the windowed knot search; any order is accepted). */
 typedef struct rbx_target {
   const float* wave;
   const float* dwave;
   int32_t n;
   float wmin, wmax;
+  int32_t ascending;
 } rbx_target;
 
 /* Per-star inputs of the fused path (pointers to (n,) arrays unless noted).

@@ -27,7 +27,7 @@ class RbxSsp(ctypes.Structure):
 
 class RbxTarget(ctypes.Structure):
     _fields_ = [("wave", c_void_p), ("dwave", c_void_p), ("n", c_int32),
-                ("wmin", c_float), ("wmax", c_float)]
+                ("wmin", c_float), ("wmax", c_float), ("ascending", c_int32)]
 
 
 class RbxStars(ctypes.Structure):

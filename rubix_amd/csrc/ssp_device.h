@@ -59,9 +59,17 @@ __device__ inline float doppler_factor(float v, float c) {
   return float(exp(double(v / c)));
 }
 
-// Shifted knot, rounded like the reference's w = fl(wave * a) (no FMA).
+// Correctly rounded float product that the compiler may not fuse into an
+// FMA with a later add (HIP's __fmul_rn is a plain, contractable multiply).
+__device__ inline float mul_nocontract(float x, float y) {
+#pragma clang fp contract(off)
+  return x * y;
+}
+
+// Shifted knot, rounded like the reference's w = fl(wave * a): the product
+// is rounded before any use (x - w must not become fma(-wave, a, x)).
 __device__ inline float knot(const SspView& g, int k, float a) {
-  return __fmul_rn(g.wave[k], a);
+  return mul_nocontract(g.wave[k], a);
 }
 
 // Number of knots with fl(wave[k] * a) <= x, minus one (in [-1, n-1]):

@@ -158,12 +158,14 @@ class TargetGrid:
         self.wmin = float(w.min())
         self.wmax = float(w.max())
         self.ascending = bool(torch.all(w[1:] > w[:-1])) if w.numel() > 1 else True
+        self.nondecreasing = bool(torch.all(w[1:] >= w[:-1])) if w.numel() > 1 else True
         s = RbxTarget()
         s.wave = w.data_ptr()
         s.dwave = self.dwave.data_ptr()
         s.n = w.numel()
         s.wmin = self.wmin
         s.wmax = self.wmax
+        s.ascending = int(self.nondecreasing)
         self._struct = s
 
     @property
