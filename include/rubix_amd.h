@@ -69,8 +69,13 @@ int rbx_spaxel_sort(const int32_t* ids, int64_t n, int32_t n_keys,
  * as rubix/core/ifu.py:261 computes it.  log_axes != 0 means ax_met/ax_age
  * hold log10 values and queries are log10'd before interpolation.
  * rbx_ssp_prepare fills cum (double, n_met*n_age*n_wave: per-template
- * prefix sums of flux[k]*(wave_obs[k]-wave_obs[k-1]), 0 at k = 0) and
- * locate (int32, n_locate: last knot index <= locate_x0 + m/locate_inv_h).
+ * prefix sums of flux[k]*(wave_obs[k]-wave_obs[k-1]), 0 at k = 0) and the
+ * knot locator, n_locate int32 pairs.  With h = 1/locate_inv_h, entry m >= 1
+ * holds k = (number of wave_obs <= locate_x0 + (m-1) h) - 1 and the bits of
+ * wave_obs[k+1] as float (+inf past the last knot); entry 0 is {-1,
+ * wave_obs[0]} (queries below locate_x0).  When h is at most the smallest
+ * knot spacing every cell holds at most one knot and locate_multi = 0;
+ * otherwise locate_multi = 1 and the kernels walk from the entry.
  */
 typedef struct rbx_ssp {
   const float* flux;
@@ -82,7 +87,7 @@ typedef struct rbx_ssp {
   int32_t n_met, n_age, n_wave, n_locate;
   float locate_x0, locate_inv_h;
   int32_t log_axes;
-  int32_t reserved;
+  int32_t locate_multi;
 } rbx_ssp;
 
 int rbx_ssp_prepare(const rbx_ssp* ssp, double* cum_out, int32_t* locate_out,

@@ -21,7 +21,7 @@ class RbxSsp(ctypes.Structure):
         ("wave_obs", c_void_p), ("cum", c_void_p), ("locate", c_void_p),
         ("n_met", c_int32), ("n_age", c_int32), ("n_wave", c_int32), ("n_locate", c_int32),
         ("locate_x0", c_float), ("locate_inv_h", c_float),
-        ("log_axes", c_int32), ("reserved", c_int32),
+        ("log_axes", c_int32), ("locate_multi", c_int32),
     ]
 
 

@@ -15,23 +15,23 @@
 //     star_prep   (thread / star, sorted order): cell, weights, a, the
 //                 in-range window [kl, kh] with the exact float predicate,
 //                 and T in O(1) from per-template prefix sums (cum).
-//     star_norm   (wavefront / star): Nn by a fixed-order wavefront reduction,
-//                 then sf = nan_to_num(T / Nn).
+//     star_norm   (wavefront / (star, lambda slab)): partial Nn of the slab's
+//                 bins; star_sf adds the 8 slabs in order, sf = nan_to_num(T/Nn).
 //     cube_accum  (wavefront / (spaxel chunk, 64-bin lambda tile)): lanes own
 //                 target bins; the wavefront walks the chunk's stars in sorted
 //                 order accumulating P_j * sf in registers.  No atomics.
-//                 Workgroups are mapped so that each XCD works on its own
-//                 lambda slab (its L2 keeps that slab of the templates).
 //     cube_combine: cube[p, j] = sum of the spaxel's chunk partials in chunk
 //                 order (fixed order => bitwise deterministic).
-//   Adjoint (star_bwd, wavefront / star): one pass over the target bins
-//   gathers dL/dcube[spaxel_i, :] and accumulates, per template corner c,
-//   V_c = sum_j g_j R_c(j), A_c = sum_j g_j dR_c(j)/da and the same with the
-//   weights dtarget_j, where R_c(j) is template row c interpolated at
-//   target_j.  Every per-star gradient (mass, age, metallicity, v_los)
-//   follows in closed form from these 16 sums, T, Nn and the prefix-sum
-//   window (closed form at the end of star_bwd_kernel).  Spectra are
-//   recomputed, never stored.
+//   The slab-mapped kernels put lambda slab b % 8 on XCD b % 8, so each
+//   XCD's L2 holds only its slab of the templates.
+//   Adjoint: star_bwd (wavefront / (star, lambda slab)) gathers
+//   dL/dcube[spaxel_i, :] over the slab's bins and accumulates, per template
+//   corner c, V_c = sum_j g_j R_c(j), A_c = sum_j g_j dR_c(j)/da and the same
+//   with the weights dtarget_j, where R_c(j) is template row c interpolated
+//   at target_j; star_grad adds the slabs in order and forms every per-star
+//   gradient (mass, age, metallicity, v_los) in closed form from these 16
+//   sums, T, Nn and the prefix-sum window.  Spectra are recomputed, never
+//   stored.
 #include <float.h>
 #include <math.h>
 
@@ -117,181 +117,167 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
   }
 }
 
-// ------------------------------------------------------------ knot windows
+// ------------------------------------------------------------ segments
 //
-// The hot loops evaluate P_j = jnp.interp(target_j, w, S) for the 64 target
-// bins of a tile.  Target bins are increasing, so the knots their segments
-// use form a short window [kb, kb + nk) (about 0.8 knots per bin at the
-// benchmark shapes).  Lane t loads knot kb + t once (its shifted wavelength
-// and its template values); every target lane then finds its segment with a
-// 6-step binary search over the wave's knots (ds_bpermute) and fetches the
-// two knots it needs the same way.  Per tile that is one template load per
-// corner per lane instead of two per target, and no data-dependent walks.
-// Tiles whose window does not fit 64 knots, or targets that are not
-// increasing, take the per-target path (segment() + direct loads).
+// The hot loops evaluate P_j = jnp.interp(target_j, w, S) with one lane per
+// target bin.  The segment starts from one probe of the locator table: the
+// cell of y = x / a gives k0 = (knots <= cell start) - 1 and the next knot,
+// and k = k0 + (next <= y).  That guess can be one knot off when x lies
+// within rounding distance of a knot, so it is then corrected with the exact
+// float32 predicate on the shifted knots w = fl(wave * a) that are loaded
+// anyway: step right while x >= w_r, left while x < w_l.  The result is
+// searchsorted(w, x, 'right') - 1 exactly, as the reference computes it
+// (the segment decides dP/da, which jumps at a knot).  The correction loop
+// also covers coarse tables (cells with several knots).
 
-struct Win {
-  int kb, nk;  // first knot, number of knots
-  bool ok;     // window valid and <= 64 knots (uniform over the wave)
-  int k;       // this lane's knot index
-  float w;     // fl(wave_obs[k] * a)
+// Element idx of an array with a wave-uniform base: the byte offset is formed
+// in 32 bits, so the load can use a scalar base plus a 32-bit vector offset
+// (every array read this way is below 4 GiB: checked by validate()).
+template <class T>
+__device__ inline T ld_at(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * uint32_t(sizeof(T)));
+}
+
+// Elements idx and idx + 1 in one 8-byte load (4-byte alignment is enough
+// for global memory on this target).
+struct alignas(4) Pair {
+  float lo, hi;
 };
 
-__device__ inline int table_guess(const SspView& g, float x, float inv_a) {
-  const float fm = (x * inv_a - g.loc_x0) * g.loc_inv_h;
-  int m = 0;
-  if (fm > 0.f) m = fm < float(g.n_loc - 1) ? int(fm) : g.n_loc - 1;
-  return g.loc[m];
+__device__ inline Pair ld_pair(const float* base, uint32_t idx) {
+  return *reinterpret_cast<const Pair*>(reinterpret_cast<const char*>(base) + idx * 4u);
 }
 
-// Window for targets in [x_first, x_last] (uniform values).  The locator
-// table bounds the first/last needed knot within a cell; margins of 2 and 3
-// knots absorb the rounding of x * inv_a, and the result is verified with
-// the exact predicate: knot kb must be <= x_first (or kb = 0) and knot ke
-// must be > x_last (or ke = n - 1), which makes the windowed segment search
-// identical to searchsorted over all knots.
-__device__ inline Win make_window(const SspView& g, float x_first, float x_last, float a,
-                                  float inv_a, bool ascending) {
-  Win W;
-  const int n = g.n_wave;
-  int kb = table_guess(g, x_first, inv_a) - 2;
-  int ke = table_guess(g, x_last, inv_a) + 3;
-  kb = kb < 0 ? 0 : (kb > n - 2 ? n - 2 : kb);
-  ke = ke > n - 1 ? n - 1 : ke;
-  if (ke < kb + 1) ke = kb + 1;
-  W.kb = kb;
-  W.nk = ke - kb + 1;
-  const int lane = threadIdx.x & 63;
-  W.k = kb + (lane < W.nk ? lane : W.nk - 1);
-  if (W.k > n - 1) W.k = n - 1;
-  W.w = knot(g, W.k, a);
-  const float w_first = __shfl(W.w, 0, 64);
-  const float w_last = __shfl(W.w, (W.nk - 1) & 63, 64);
-  W.ok = ascending && W.nk <= 64 && (kb == 0 || w_first <= x_first) &&
-         (ke == n - 1 || w_last > x_last);
-  return W;
+// Per-star constants of the search: a, 1/a and the locator cell of target x
+// as x * c1 + c0.
+struct Shift {
+  float a, inv_a, c1, c0;
+};
+
+__device__ inline Shift shift_of(const SspView& g, float a, float inv_a) {
+  Shift h;
+  h.a = a;
+  h.inv_a = inv_a;
+  h.c1 = inv_a * g.loc_inv_h;
+  h.c0 = 1.0f - g.loc_x0 * g.loc_inv_h;
+  return h;
 }
 
-// Number of window knots <= x (window sorted; branch-free binary search).
-__device__ inline int win_count(const Win& W, float x) {
-  int cnt = 0;
-#pragma unroll
-  for (int step = 32; step >= 1; step >>= 1) {
-    const int probe = cnt + step - 1;
-    const float wp = __shfl(W.w, probe & 63, 64);
-    if (probe < W.nk && wp <= x) cnt += step;
-  }
-  return cnt;
-}
-
-// jnp.interp segment of target x from the window: local knot indices
-// li-1, li of the segment, interpolation weight and clamp side.
-struct WinSeg {
-  int li;
-  int side;
+// jnp.interp segment [i0, i0 + 1] of target x: interpolation weight theta
+// (0 / 1 at the left / right clamp, 0 for a zero-width segment, as
+// jnp.interp's dx == 0 branch) and, for the adjoint, kappa = lambda(x) /
+// (w_r - w_l) with lambda(x) = l0 + theta (l1 - l0) on the unshifted knots,
+// so that dP/da = -kappa (S1 - S0) inside the grid (0 at the clamps).
+struct Knots {
+  uint32_t i0;
   float theta;
-  float wl, wr;
+  float kappa;
 };
 
-__device__ inline WinSeg win_segment(const SspView& g, const Win& W, float x) {
+template <bool kKappa>
+__device__ inline Knots knots_of(const SspView& g, float x, const Shift& h) {
   const int n = g.n_wave;
-  const int kstar = W.kb - 1 + win_count(W, x);  // last knot <= x, in [-1, n-1]
-  int i = kstar + 1;
-  i = i < 1 ? 1 : (i > n - 1 ? n - 1 : i);
-  WinSeg s;
-  s.li = i - W.kb;  // in [1, nk - 1] for a valid window
-  s.wl = __shfl(W.w, (s.li - 1) & 63, 64);
-  s.wr = __shfl(W.w, s.li & 63, 64);
-  s.side = kstar < 0 ? -1 : ((kstar >= n - 1 && x > s.wr) ? 1 : 0);
-  s.theta = (x - s.wl) / (s.wr - s.wl);
-  return s;
-}
-
-// P_j = jnp.interp(target_j, w, S) with S = m * sum_c w_c f_c, per-target
-// path (direct loads of both knots).
-__device__ inline float interp_value(const SspView& g, const StarState& s, const Seg& sg) {
-  const int nw = g.n_wave;
-  const float* f00 = g.flux + int64_t(s.row) * nw;
-  const float* f10 = f00 + int64_t(g.n_age) * nw;
-  const float* f01 = f00 + nw;
-  const float* f11 = f10 + nw;
-  const int i0 = sg.i - 1, i1 = sg.i;
-  const float F0 = s.w[0] * f00[i0] + s.w[1] * f10[i0] + s.w[2] * f01[i0] + s.w[3] * f11[i0];
-  const float F1 = s.w[0] * f00[i1] + s.w[1] * f10[i1] + s.w[2] * f01[i1] + s.w[3] * f11[i1];
-  const float S0 = F0 * s.m, S1 = F1 * s.m;
-  float P = S0 + sg.theta * (S1 - S0);
-  if (sg.side < 0) P = S0;
-  if (sg.side > 0) P = S1;
-  return P;
-}
-
-// P for this lane's target x in the tile spanning [x_first, x_last].
-__device__ inline float forward_value(const SspView& g, const StarState& s, float x,
-                                      float x_first, float x_last, bool ascending) {
-  const Win W = make_window(g, x_first, x_last, s.a, s.inv_a, ascending);
-  if (W.ok) {
-    const int nw = g.n_wave;
-    const float* f00 = g.flux + int64_t(s.row) * nw;
-    const float* f10 = f00 + int64_t(g.n_age) * nw;
-    const float* f01 = f00 + nw;
-    const float* f11 = f10 + nw;
-    const int k = W.k;
-    const float Fk = s.w[0] * f00[k] + s.w[1] * f10[k] + s.w[2] * f01[k] + s.w[3] * f11[k];
-    const WinSeg sg = win_segment(g, W, x);
-    const float S0 = __shfl(Fk, (sg.li - 1) & 63, 64) * s.m;
-    const float S1 = __shfl(Fk, sg.li & 63, 64) * s.m;
-    float P = S0 + sg.theta * (S1 - S0);
-    if (sg.side < 0) P = S0;
-    if (sg.side > 0) P = S1;
-    return P;
+  float fm = fmaf(x, h.c1, h.c0);
+  fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
+  const int2 e = ld_at(reinterpret_cast<const int2*>(g.loc), uint32_t(fm));
+  const float y = x * h.inv_a;
+  const int k = e.x + (__int_as_float(e.y) <= y ? 1 : 0);
+  int i = k + 1 < 1 ? 1 : (k + 1 > n - 1 ? n - 1 : k + 1);
+  Pair lp = ld_pair(g.wave, uint32_t(i - 1));
+  float wl = mul_nocontract(lp.lo, h.a), wr = mul_nocontract(lp.hi, h.a);
+  // exact segment: i in [1, n-1] moves monotonically, so the loop ends
+  while ((x >= wr && i < n - 1) || (x < wl && i > 1)) {
+    i += x >= wr ? 1 : -1;
+    lp = ld_pair(g.wave, uint32_t(i - 1));
+    wl = mul_nocontract(lp.lo, h.a);
+    wr = mul_nocontract(lp.hi, h.a);
   }
-  const Seg sg = segment(g, x, s.a, s.inv_a);
-  return interp_value(g, s, sg);
+  const uint32_t i0 = uint32_t(i - 1);
+  const float l0 = lp.lo, l1 = lp.hi;
+  const float dx = wr - wl;
+  const float r = __builtin_amdgcn_rcpf(dx);
+  const bool flat = !(dx > 0.f);
+  const float th = flat ? 0.f : (x - wl) * r;
+  const bool left = (i == 1) && (x < wl);
+  const bool right = (i == n - 1) && (x > wr);
+  Knots K;
+  K.i0 = i0;
+  K.theta = left ? 0.f : (right ? 1.f : th);
+  K.kappa = 0.f;
+  if (kKappa) K.kappa = (left || right || flat) ? 0.f : (l0 + th * (l1 - l0)) * r;
+  return K;
 }
 
-struct TileX {
-  int j;
-  bool valid;
-  float x, x_first, x_last;
+// Row pointers of a star's four template corners (uniform over the wave).
+struct Rows {
+  const float* f[4];
 };
 
-__device__ inline TileX tile_x(const rbx_target& tg, int tile) {
-  TileX t;
-  const int j0 = tile * kTile;
-  t.j = j0 + (threadIdx.x & 63);
-  t.valid = t.j < tg.n;
-  t.x = tg.wave[t.valid ? t.j : tg.n - 1];
-  t.x_first = tg.wave[j0];
-  t.x_last = tg.wave[j0 + kTile - 1 < tg.n ? j0 + kTile - 1 : tg.n - 1];
+__device__ inline Rows rows_of(const SspView& g, int row) {
+  Rows R;
+  const int nw = g.n_wave;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) R.f[c] = g.flux + int64_t(corner_row(row, c, g.n_age)) * nw;
+  return R;
+}
+
+// P = S0 + theta (S1 - S0), S = m * sum_c w_c f_c (the spectrum of
+// calculate_spectra * mass at the two knots).
+__device__ inline float star_value(const Rows& R, const StarState& s, const Knots& K) {
+  float F0 = 0.f, F1 = 0.f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const Pair f = ld_pair(R.f[c], K.i0);
+    F0 += s.w[c] * f.lo;
+    F1 += s.w[c] * f.hi;
+  }
+  const float S0 = F0 * s.m, S1 = F1 * s.m;
+  return S0 + K.theta * (S1 - S0);
+}
+
+// Wave and star of this thread in the slab-mapped kernels: workgroup b runs
+// on XCD b % 8 and handles lambda slab b % 8 of four stars, one per
+// wavefront, so each XCD's L2 holds only its slab of the templates.
+struct SlabTask {
+  int slab, wave, lane;
+  int64_t r;
+};
+
+__device__ inline SlabTask slab_task() {
+  SlabTask t;
+  t.slab = blockIdx.x & (kSlabs - 1);
+  t.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  t.lane = threadIdx.x & 63;
+  t.r = int64_t(blockIdx.x >> 3) * 4 + t.wave;
   return t;
 }
 
 // ------------------------------------------------------------ norm
 
-// Slab-mapped: workgroup b runs on XCD b % 8 and handles lambda slab b % 8
-// (tiles [slab * tps, slab * tps + tps)) of four stars, one per wavefront, so
-// each XCD's L2 holds only its slab of the templates.  Writes the slab's
-// partial Nn; star_sf_kernel adds the slabs in a fixed order.
-__global__ void __launch_bounds__(256) star_norm_kernel(SspView g, rbx_target tg, int ascending,
-                                                        int64_t n, const StarState* __restrict__ state,
-                                                        int n_tiles, int tps,
+// Partial Nn of (star, slab); star_sf_kernel adds the slabs in a fixed order.
+__global__ void __launch_bounds__(256) star_norm_kernel(SspView g, rbx_target tg, int64_t n,
+                                                        const StarState* __restrict__ state,
+                                                        int slab_bins,
                                                         float* __restrict__ nn_part) {
-  const int slab = blockIdx.x & 7;
-  const int64_t r = int64_t(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);
-  if (r >= n) return;
-  const StarState s = state[r];
+  const SlabTask t = slab_task();
+  if (t.r >= n) return;
+  const StarState s = state[t.r];
+  if (s.row < 0) {
+    if (t.lane == 0) nn_part[t.r * kSlabs + t.slab] = 0.f;
+    return;
+  }
+  const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
+  const Shift h = shift_of(g, s.a, s.inv_a);
+  const uint32_t j0 = uint32_t(t.slab * slab_bins);
+  const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
   float acc = 0.f;
-  if (s.row >= 0) {
-    const int t0 = slab * tps;
-    const int t1 = t0 + tps < n_tiles ? t0 + tps : n_tiles;
-    for (int t = t0; t < t1; ++t) {
-      const TileX tx = tile_x(tg, t);
-      const float P = forward_value(g, s, tx.x, tx.x_first, tx.x_last, ascending != 0);
-      if (tx.valid) acc += P * tg.dwave[tx.j];
-    }
+#pragma unroll 2
+  for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
+    const Knots K = knots_of<false>(g, ld_at(tg.wave, j), h);
+    acc += star_value(R, s, K) * ld_at(tg.dwave, j);
   }
   acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) nn_part[r * kSlabs + slab] = acc;
+  if (t.lane == 0) nn_part[t.r * kSlabs + t.slab] = acc;
 }
 
 __global__ void star_sf_kernel(int64_t n, const float* __restrict__ nn_part,
@@ -342,7 +328,7 @@ __global__ void chunk_fill_kernel(const int32_t* __restrict__ offsets, int32_t n
 // ------------------------------------------------------------ accumulate
 
 __global__ void __launch_bounds__(64) cube_accum_kernel(
-    SspView g, rbx_target tg, int ascending, const StarState* __restrict__ state,
+    SspView g, rbx_target tg, const StarState* __restrict__ state,
     const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr,
     int n_tiles, int tiles_per_xcd, double* __restrict__ partial) {
   const int b = blockIdx.x;
@@ -352,18 +338,22 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
   const int chunk = q / tiles_per_xcd;
   if (tile >= n_tiles || chunk >= *n_chunks_ptr) return;
   const ChunkDesc d = desc[chunk];
-  const TileX tx = tile_x(tg, tile);
+  const int j = tile * kTile + int(threadIdx.x);
+  const bool valid = j < tg.n;
+  const float x = ld_at(tg.wave, uint32_t(valid ? j : tg.n - 1));
   // double accumulator: the cube is a sum of up to chunk-size positive
   // terms per lane; accumulating in double keeps the forward's rounding at
   // the level of the individual float32 terms.
   double acc = 0.0;
+#pragma unroll 2
   for (int r = d.begin; r < d.end; ++r) {
     const StarState s = state[r];
     if (s.row < 0) continue;
-    const float P = forward_value(g, s, tx.x, tx.x_first, tx.x_last, ascending != 0);
-    acc += double(P * s.sf);
+    const Rows R = rows_of(g, s.row);
+    const Knots K = knots_of<false>(g, x, shift_of(g, s.a, s.inv_a));
+    acc += double(star_value(R, s, K) * s.sf);
   }
-  if (tx.valid) partial[int64_t(chunk) * tg.n + tx.j] = acc;
+  if (valid) partial[int64_t(chunk) * tg.n + j] = acc;
 }
 
 __global__ void cube_combine_kernel(const double* __restrict__ partial,
@@ -384,110 +374,71 @@ __global__ void cube_combine_kernel(const double* __restrict__ partial,
 
 // Per-slab sums of one star (see the file comment): V = sum_j w_j R_c(j) and
 // A = sum_j w_j dR_c(j)/da for the weights w = dL/dC (g) and dtarget (d).
-// The Doppler sums (A) cancel strongly (sum_j g_j dP_j/da is a small
-// difference of terms ~ slope * lambda / dlambda; a float32 autograd of the
-// reference graph loses ~3e-4 on d/dv_los), so they are accumulated in
-// double; the value sums (V) do not cancel and stay float32.
-struct BwdSums {
-  float Vg[4], Vd[4];
-  double Ag[4], Ad[4];
-};
+// Each lane accumulates its bins in float32; the wave's 64 lane sums are
+// added in double in a fixed order (lds_sum16), so the Doppler sums A, which
+// cancel strongly across bins, lose nothing beyond the float32 per-bin terms.
+constexpr int kRedStride = 17;  // padded LDS row of one lane's 16 sums
 
-__device__ inline void bwd_accumulate(BwdSums& S, float gj, float dj, float theta, int side,
-                                      float kappa, const float f0[4], const float f1[4]) {
-  const double gk = side == 0 ? -double(gj) * double(kappa) : 0.0;
-  const double dk = side == 0 ? -double(dj) * double(kappa) : 0.0;
+// v[0..15] of all 64 lanes -> lane 4q holds the wave total of v[q] (double).
+// The order is fixed: 16 consecutive lanes in order, then pairs of those.
+__device__ inline double lds_sum16(const float v[16], float* buf, int lane) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const float df = f1[c] - f0[c];
-    float R = f0[c] + theta * df;
-    if (side < 0) R = f0[c];
-    if (side > 0) R = f1[c];
-    S.Vg[c] += gj * R;
-    S.Vd[c] += dj * R;
-    S.Ag[c] += gk * double(df);
-    S.Ad[c] += dk * double(df);
-  }
+  for (int q = 0; q < 16; ++q) buf[lane * kRedStride + q] = v[q];
+  __syncthreads();
+  const int q = lane >> 2, p = lane & 3;
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc += double(buf[(p * 16 + t) * kRedStride + q]);
+  acc += __shfl_xor(acc, 1, 64);
+  acc += __shfl_xor(acc, 2, 64);
+  return acc;
 }
 
 // Slab-mapped like star_norm_kernel; writes the 16 sums of (star, slab) to
-// vpart / apart for star_grad_kernel.
+// vpart (value sums, float) / apart (Doppler sums, double).
 __global__ void __launch_bounds__(256) star_bwd_kernel(
-    SspView g, rbx_target tg, int ascending, int64_t n, const StarState* __restrict__ state,
-    const float* __restrict__ dcube, int n_tiles, int tps, float* __restrict__ vpart,
+    SspView g, rbx_target tg, int64_t n, const StarState* __restrict__ state,
+    const float* __restrict__ dcube, int slab_bins, float* __restrict__ vpart,
     double* __restrict__ apart) {
-  const int slab = blockIdx.x & 7;
-  const int64_t r = int64_t(blockIdx.x >> 3) * 4 + (threadIdx.x >> 6);
-  if (r >= n) return;
-  const StarState s = state[r];
-  BwdSums S;
+  __shared__ float red[4][64 * kRedStride];
+  const SlabTask t = slab_task();
+  const bool live = t.r < n;
+  float v[16];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    S.Vg[c] = 0.f; S.Vd[c] = 0.f; S.Ag[c] = 0.0; S.Ad[c] = 0.0;
-  }
+  for (int q = 0; q < 16; ++q) v[q] = 0.f;
+  StarState s;
+  s.row = -1;
+  if (live) s = state[t.r];
   if (s.row >= 0) {
-    const int nw = g.n_wave;
-    const float* fr[4];
-    for (int c = 0; c < 4; ++c) fr[c] = g.flux + int64_t(corner_row(s.row, c, g.n_age)) * nw;
+    const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
     const float* grow = dcube + int64_t(s.spaxel) * tg.n;
-    const int t0 = slab * tps;
-    const int t1 = t0 + tps < n_tiles ? t0 + tps : n_tiles;
-    for (int t = t0; t < t1; ++t) {
-      const TileX tx = tile_x(tg, t);
-      const float gj = tx.valid ? grow[tx.j] : 0.f;
-      const float dj = tx.valid ? tg.dwave[tx.j] : 0.f;
-      const Win W = make_window(g, tx.x_first, tx.x_last, s.a, s.inv_a, ascending != 0);
-      float f0[4], f1[4];
-      float theta, kappa;
-      int side;
-      if (W.ok) {
-        float fk[4];
+    const Shift h = shift_of(g, s.a, s.inv_a);
+    const uint32_t j0 = uint32_t(t.slab * slab_bins);
+    const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
+#pragma unroll 2
+    for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
+      const float gj = ld_at(grow, j), dj = ld_at(tg.dwave, j);
+      const Knots K = knots_of<true>(g, ld_at(tg.wave, j), h);
+      const float gk = -gj * K.kappa, dk = -dj * K.kappa;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) fk[c] = fr[c][W.k];
-        const float lam = g.wave[W.k];
-        const WinSeg sg = win_segment(g, W, tx.x);
-        const int l0 = (sg.li - 1) & 63, l1 = sg.li & 63;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          f0[c] = __shfl(fk[c], l0, 64);
-          f1[c] = __shfl(fk[c], l1, 64);
-        }
-        const float lam0 = __shfl(lam, l0, 64), lam1 = __shfl(lam, l1, 64);
-        theta = sg.theta;
-        side = sg.side;
-        kappa = (lam0 + theta * (lam1 - lam0)) / (sg.wr - sg.wl);
-      } else {
-        const Seg sg = segment(g, tx.x, s.a, s.inv_a);
-        const int i0 = sg.i - 1, i1 = sg.i;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          f0[c] = fr[c][i0];
-          f1[c] = fr[c][i1];
-        }
-        theta = sg.theta;
-        side = sg.side;
-        kappa = (g.wave[i0] + theta * (g.wave[i1] - g.wave[i0])) / (sg.wr - sg.wl);
+      for (int c = 0; c < 4; ++c) {
+        const Pair f = ld_pair(R.f[c], K.i0);
+        const float f0 = f.lo, f1 = f.hi;
+        const float df = f1 - f0;
+        const float Rc = f0 + K.theta * df;
+        v[c] += gj * Rc;
+        v[4 + c] += dj * Rc;
+        v[8 + c] += gk * df;
+        v[12 + c] += dk * df;
       }
-      bwd_accumulate(S, gj, dj, theta, side, kappa, f0, f1);
     }
   }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    S.Vg[c] = wave_sum(S.Vg[c]);
-    S.Vd[c] = wave_sum(S.Vd[c]);
-    S.Ag[c] = wave_sum(S.Ag[c]);
-    S.Ad[c] = wave_sum(S.Ad[c]);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    float* vp = vpart + (r * kSlabs + slab) * 8;
-    double* ap = apart + (r * kSlabs + slab) * 8;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      vp[c] = S.Vg[c];
-      vp[4 + c] = S.Vd[c];
-      ap[c] = S.Ag[c];
-      ap[4 + c] = S.Ad[c];
-    }
+  const double tot = lds_sum16(v, red[t.wave], t.lane);
+  const int q = t.lane >> 2;
+  if (live && (t.lane & 3) == 0) {
+    const int64_t base = (t.r * kSlabs + t.slab) * 8;
+    if (q < 8) vpart[base + q] = float(tot);
+    else apart[base + q - 8] = tot;
   }
 }
 
@@ -497,7 +448,7 @@ __global__ void __launch_bounds__(256) star_bwd_kernel(
 //   dT/du = m a W_u, dT/ds = m a W_s, dT/da = T / a, dT/dm = T / m,
 //   dL/dtheta = sf sum_j g_j dP_j/dtheta + sbar dsf/dtheta, sbar = sum_j g_j P_j
 // (dsf/dm = 0; dsf = 0 where T / Nn is not finite, as nan_to_num's gradient).
-__global__ void star_grad_kernel(SspView g, rbx_stars st, const StarState* __restrict__ state,
+__global__ void __launch_bounds__(256) star_grad_kernel(SspView g, rbx_stars st, const StarState* __restrict__ state,
                                  const float* __restrict__ vpart, const double* __restrict__ apart,
                                  float* __restrict__ dmass, float* __restrict__ dage,
                                  float* __restrict__ dmet, float* __restrict__ dvlos) {
@@ -587,16 +538,24 @@ __global__ void __launch_bounds__(256) ssp_cum_kernel(const float* __restrict__ 
   }
 }
 
+// Locator entries (layout in the rbx_ssp comment of the C header): entry 0
+// = {-1, wave[0]}; entry m >= 1 = {k, wave[k + 1]} with k the last knot
+// <= x0 + (m - 1) h (double comparison), +inf after the last knot.
 __global__ void ssp_locate_kernel(const float* __restrict__ wave, int nw, float x0, float inv_h,
                                   int nloc, int32_t* __restrict__ loc) {
   for (int64_t m = gtid(); m < nloc; m += int64_t(gridDim.x) * blockDim.x) {
-    const double xm = double(x0) + double(m) / double(inv_h);
-    int lo = 0, hi = nw;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (double(wave[mid]) <= xm) lo = mid + 1; else hi = mid;
+    int k = -1;
+    if (m > 0) {
+      const double xm = double(x0) + double(m - 1) / double(inv_h);
+      int lo = 0, hi = nw;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (double(wave[mid]) <= xm) lo = mid + 1; else hi = mid;
+      }
+      k = lo - 1;
     }
-    loc[m] = lo - 1;
+    loc[2 * m] = k;
+    loc[2 * m + 1] = __float_as_int(k + 1 < nw ? wave[k + 1] : INFINITY);
   }
 }
 
@@ -679,6 +638,8 @@ static int validate(const rbx_ssp* ssp, const rbx_target* tg, const rbx_stars* s
     return fail(RBX_EINVAL, "rbx_cube: null star array");
   if (st->vel_comp < 0 || st->vel_comp >= st->vel_stride)
     return fail(RBX_EINVAL, "rbx_cube: vel_comp out of range");
+  if (tg->n >= (1 << 28) || ssp->n_wave >= (1 << 28) || ssp->n_locate >= (1 << 27))
+    return fail(RBX_EINVAL, "rbx_cube: wavelength axes or locator table too long");
   if (st->n_spaxel < 1 || st->n_keys < st->n_spaxel)
     return fail(RBX_EINVAL, "rbx_cube: need 1 <= n_spaxel <= n_keys");
   if (!(st->speed_of_light > 0.f)) return fail(RBX_EINVAL, "rbx_cube: speed_of_light must be > 0");
@@ -753,8 +714,8 @@ int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* 
     const int tps = (n_tiles + kSlabs - 1) / kSlabs;
     const int64_t blocks = int64_t(kSlabs) * ((n + 3) / 4);
     if (blocks > INT32_MAX) return fail(RBX_EINVAL, "rbx_cube_fwd: too many stars for one launch");
-    hipLaunchKernelGGL(star_norm_kernel, dim3(blocks), dim3(256), 0, s, g, *target,
-                       target->ascending, n, L.state, n_tiles, tps, L.nn_part);
+    hipLaunchKernelGGL(star_norm_kernel, dim3(blocks), dim3(256), 0, s, g, *target, n, L.state,
+                       tps * kTile, L.nn_part);
     hipLaunchKernelGGL(star_sf_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L.nn_part,
                        L.state);
   }
@@ -767,9 +728,8 @@ int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* 
   if (n > 0) {
     const int64_t blocks = int64_t(8) * tpx * max_chunks;
     if (blocks > INT32_MAX) return fail(RBX_EINVAL, "rbx_cube_fwd: problem too large for one launch");
-    hipLaunchKernelGGL(cube_accum_kernel, dim3(blocks), dim3(64), 0, s, g, *target,
-                       target->ascending, L.state, L.desc, L.chunk_off + ns, n_tiles, tpx,
-                       L.partial);
+    hipLaunchKernelGGL(cube_accum_kernel, dim3(blocks), dim3(64), 0, s, g, *target, L.state,
+                       L.desc, L.chunk_off + ns, n_tiles, tpx, L.partial);
   }
   combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s);
   return check_launch("rbx_cube_fwd");
@@ -795,8 +755,8 @@ int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* 
   const int tps = (n_tiles + kSlabs - 1) / kSlabs;
   const int64_t blocks = int64_t(kSlabs) * ((n + 3) / 4);
   if (blocks > INT32_MAX) return fail(RBX_EINVAL, "rbx_cube_bwd: too many stars for one launch");
-  hipLaunchKernelGGL(star_bwd_kernel, dim3(blocks), dim3(256), 0, s, g, *target,
-                     target->ascending, n, L.state, dcube, n_tiles, tps, L.vpart, L.apart);
+  hipLaunchKernelGGL(star_bwd_kernel, dim3(blocks), dim3(256), 0, s, g, *target, n, L.state,
+                     dcube, tps * kTile, L.vpart, L.apart);
   hipLaunchKernelGGL(star_grad_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *stars,
                      L.state, L.vpart, L.apart, dmass, dage, dmet, dvlos);
   return check_launch("rbx_cube_bwd");

@@ -37,7 +37,7 @@ struct SspView {
   const float* __restrict__ ax_age;
   const float* __restrict__ wave;
   const double* __restrict__ cum;
-  const int32_t* __restrict__ loc;
+  const int32_t* __restrict__ loc;  // n_loc pairs {k, bits of wave[k + 1]}
   int n_met, n_age, n_wave, n_loc;
   float loc_x0, loc_inv_h;
   int log_axes;
@@ -72,15 +72,20 @@ __device__ inline float knot(const SspView& g, int k, float a) {
   return mul_nocontract(g.wave[k], a);
 }
 
+// Locator cell of an unshifted wavelength y (cell 0 = below the first knot;
+// NaN maps to cell 0).
+__device__ inline int loc_cell(const SspView& g, float y) {
+  float fm = (y - g.loc_x0) * g.loc_inv_h + 1.0f;
+  fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
+  return int(fm);
+}
+
 // Number of knots with fl(wave[k] * a) <= x, minus one (in [-1, n-1]):
 // searchsorted(w, x, 'right') - 1 with the exact float predicate.  The
 // locator table gives a start within a knot or two; the walks terminate
 // after at most n steps for any input.
 __device__ inline int locate(const SspView& g, float x, float a, float inv_a) {
-  const float fm = (x * inv_a - g.loc_x0) * g.loc_inv_h;
-  int m = 0;
-  if (fm > 0.f) m = fm < float(g.n_loc - 1) ? int(fm) : g.n_loc - 1;
-  int k = g.loc[m];
+  int k = g.loc[2 * loc_cell(g, x * inv_a)];
   const int n = g.n_wave;
   while (k + 1 < n && knot(g, k + 1, a) <= x) ++k;
   while (k >= 0 && knot(g, k, a) > x) --k;

@@ -105,17 +105,28 @@ class PreparedSSP:
         for ax, nm_ in ((self.ax_met, "metallicity"), (self.ax_age, "age")):
             if not bool(torch.all(ax[1:] > ax[:-1])):
                 raise ValueError(f"SSP {nm_} axis must be strictly increasing")
+        # knot locator: cells of width h <= the smallest knot spacing hold at
+        # most one knot each (one probe per query); a grid that would need more
+        # than MAX_LOCATE cells gets wider cells and the kernels walk.
         h = float(dw.min())
         span = float(w[-1] - w[0])
-        nloc = int(math.floor(span / h)) + 2
+        nloc = int(math.floor(span / h)) + 3
+        self.locate_multi = False
         if nloc > self.MAX_LOCATE:
             nloc = self.MAX_LOCATE
-            h = span / (nloc - 2)
+            h = span / (nloc - 3)
+            self.locate_multi = True
         self.n_locate = nloc
         self.locate_x0 = float(np.float32(w[0]))
-        self.locate_inv_h = float(np.float32(1.0 / h))
+        inv_h = np.float32(1.0 / h)
+        # the float32 1/h must not widen the cells past the smallest spacing
+        while not self.locate_multi and 1.0 / float(inv_h) > float(dw.min()):
+            inv_h = np.nextafter(inv_h, np.float32(np.inf))
+        self.locate_inv_h = float(inv_h)
+        if span * self.locate_inv_h + 2 > nloc - 1:
+            self.n_locate = nloc = int(math.floor(span * self.locate_inv_h)) + 3
         self.cum = torch.empty(nm * na * nw, dtype=torch.float64, device=dev)
-        self.locate = torch.empty(nloc, dtype=torch.int32, device=dev)
+        self.locate = torch.empty(2 * nloc, dtype=torch.int32, device=dev)
         self.shape = (nm, na, nw)
         self._struct = self._make_struct()
         call("rbx_ssp_prepare", self._struct, ptr(self.cum), ptr(self.locate), stream_handle(dev))
@@ -134,6 +145,7 @@ class PreparedSSP:
         s.locate_x0 = self.locate_x0
         s.locate_inv_h = self.locate_inv_h
         s.log_axes = int(self.log_axes)
+        s.locate_multi = int(self.locate_multi)
         return s
 
     @property
