@@ -114,7 +114,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n-stars", type=int, default=1_000_000)
+    ap.add_argument("--workload", choices=("single", "large", "batched"), default="single",
+                    help="single: 1M stars -> 50x50x3721 (the headline); large: 10M stars -> "
+                         "150x150x3721; batched: 64 ragged galaxies in one pass")
+    ap.add_argument("--n-stars", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=3,
@@ -131,7 +134,14 @@ def main():
               "design (sharding by stars + a lambda-slab reduce-scatter) and is not built.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "large":
+        return run_large(args)
+    if args.workload == "batched":
+        return run_batched(args)
+    return run_single(args)
 
+
+def run_single(args):
     import numpy as np
     import torch
 
@@ -140,7 +150,7 @@ def main():
 
     dev = torch.device("cuda", 0)
     t_gen = time.perf_counter()
-    w = S.config2(n_stars=args.n_stars)
+    w = S.config2(n_stars=args.n_stars or 1_000_000)
     log(f"workload generated in {time.perf_counter() - t_gen:.1f}s: N={w.n_stars}")
     g, geo = w.grid, w.geom
     nb, nt = geo["nb"], len(geo["wave_target"])
@@ -249,6 +259,225 @@ def main():
                    "n_stars": n, "n_spaxel": nb * nb, "n_target": nt, "n_source": n_src},
         "roofline": roofline, "cpu_baseline": cpu, "split": split, "plain": plain_res,
         "per_call_ms": {k: round(v, 4) for k, v in per_call.items()},
+    }
+    print(json.dumps(res), flush=True)
+
+
+
+def _chi2_weights(tgt):
+    import torch
+
+    return 1.0 / torch.clamp((tgt.abs() / 30.0) ** 2, min=(1e-6 * float(tgt.abs().max())) ** 2)
+
+
+def run_large(args):
+    """10M stars -> 150 x 150 x 3721 with the small SSP grid: fused forward +
+    adjoint of chi^2 with per-star spectra recomputed in the adjoint (149 GB
+    if stored).  Reports peak HBM, and the recompute overhead
+    t(fused adjoint) / t(stored-spectra adjoint of the plain path) at 1M
+    stars of the same geometry, where storing fits."""
+    import numpy as np
+    import torch
+
+    from rubix_amd import ops, synthetic as S
+    from rubix_amd.autograd import render, render_plain
+
+    dev = torch.device("cuda", 0)
+    n = args.n_stars or 10_000_000
+    t_gen = time.perf_counter()
+    w = S.config5(n_stars=n)
+    log(f"workload generated in {time.perf_counter() - t_gen:.1f}s: N={n}")
+    g, geo = w.grid, w.geom
+    nb, nt = geo["nb"], len(geo["wave_target"])
+    n_src = g["flux"].shape[-1]
+    ssp = ops.PreparedSSP(g["flux"], g["ax_met"], g["ax_age"], g["wave"], geo["z"], g["log_axes"], dev)
+    tg = ops.TargetGrid(geo["wave_target"], dev)
+    edges = torch.as_tensor(geo["edges"], device=dev)
+
+    def leaves(m):
+        t = {k: torch.as_tensor(v[:m], device=dev) for k, v in w.stars.items()}
+        for k in ("mass", "age", "metallicity", "velocity"):
+            t[k].requires_grad_(True)
+        sp = torch.tensor(geo["psf_sigma"], device=dev, requires_grad=True)
+        sl = torch.tensor(geo["lsf_sigma"], device=dev, requires_grad=True)
+        return t, sp, sl
+
+    def make_step(t, sp, sl, fn, tgt, wt):
+        def fwd():
+            return fn(t["coords"], t["velocity"], t["mass"], t["age"], t["metallicity"], edges,
+                      ssp, tg, nb, sp, geo["psf_size"], sl, geo["wave_res"])
+
+        def step(ev):
+            for k in ("mass", "age", "metallicity", "velocity"):
+                t[k].grad = None
+            sp.grad = None
+            sl.grad = None
+            if ev:
+                ev[0].record()
+            out = fwd()
+            loss = ((out - tgt) ** 2 * wt).sum()
+            if ev:
+                ev[1].record()
+            loss.backward()
+            if ev:
+                ev[2].record()
+        return fwd, step
+
+    t, sp, sl = leaves(n)
+    with torch.no_grad():
+        fwd0, _ = make_step(t, sp, sl, render, None, None)
+        tgt = fwd0() * 1.05
+        wt = _chi2_weights(tgt)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    _, step = make_step(t, sp, sl, render, tgt, wt)
+    elapsed, fwd_ms, adj_ms = timed_leg(step, args.warmup, args.steps)
+    ms = elapsed / args.steps * 1e3
+    peak = torch.cuda.max_memory_allocated(dev) / 1e9
+    fwd_b, adj_b = algorithmic_bytes(n, n_src, nt, nb * nb * nt)
+    del t, sp, sl, tgt, wt
+    torch.cuda.empty_cache()
+
+    # recompute overhead at 1M stars of the same geometry
+    m = min(n, 1_000_000)
+    overhead = {}
+    for name, fn in (("recompute", render), ("stored", render_plain)):
+        t, sp, sl = leaves(m)
+        with torch.no_grad():
+            fwd0, _ = make_step(t, sp, sl, render, None, None)
+            tgt = fwd0() * 1.05
+            wt = _chi2_weights(tgt)
+        _, step = make_step(t, sp, sl, fn, tgt, wt)
+        _, f, a = timed_leg(step, 1, 3)
+        overhead[name] = {"fwd_ms_median": round(float(np.median(f)), 3),
+                          "adj_ms_median": round(float(np.median(a)), 3)}
+        del t, sp, sl, tgt, wt
+        torch.cuda.empty_cache()
+    ratio = overhead["recompute"]["adj_ms_median"] / overhead["stored"]["adj_ms_median"]
+    res = {
+        "metric": "star·λ-bins/s fwd+adjoint, 10M stars→150²×3721 cube (recompute)",
+        "value": n * nt / (elapsed / args.steps), "unit": "star·λ-bins/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"single galaxy, {n} stars -> {nb}x{nb}x{nt} cube, SSP 6x20x{n_src}, "
+                               "PSF 11x11 sigma 3 px, LSF sigma 1.2 A; fused forward + adjoint of chi^2",
+                   "n_stars": n, "n_spaxel": nb * nb, "n_target": nt, "n_source": n_src},
+        "split": {"fwd_ms_median": round(float(np.median(fwd_ms)), 3),
+                  "adj_ms_median": round(float(np.median(adj_ms)), 3)},
+        "peak_hbm_gb": round(peak, 2),
+        "per_star_spectra_if_stored_gb": round(n * nt * 4 / 1e9, 1),
+        "step_effective_frac": round((fwd_b + adj_b) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+        "recompute_overhead": {"n_stars": m, "adj_ratio_recompute_over_stored": round(ratio, 4),
+                               **overhead},
+    }
+    print(json.dumps(res), flush=True)
+
+
+def run_batched(args):
+    """64 galaxies (star counts log-uniform on 5e4..2e6, spaxel grids from
+    {25, 50, 100}, shared 3721-bin axis and small SSP grid): chi^2 forward +
+    adjoint of the whole batch in one fused pass per stage, launch tuning
+    from the autotuner, the step captured in a HIP graph and replayed.
+    Compared with looping the plain per-galaxy path and the fused
+    single-galaxy path over the same galaxies."""
+    import numpy as np
+    import torch
+
+    from rubix_amd import autotune, ops, synthetic as S
+    from rubix_amd.autograd import render, render_plain
+    from rubix_amd.batch import GalaxyBatch, GraphedStep
+
+    dev = torch.device("cuda", 0)
+    t_gen = time.perf_counter()
+    ws = S.config4()
+    if args.n_stars:
+        ws = S.config4(scale=args.n_stars / sum(w.n_stars for w in ws))
+    log(f"batch generated in {time.perf_counter() - t_gen:.1f}s: "
+        f"{len(ws)} galaxies, {sum(w.n_stars for w in ws)} stars")
+    grid, geo = ws[0].grid, ws[0].geom
+    nt = len(geo["wave_target"])
+    n_src = grid["flux"].shape[-1]
+    ssp = ops.PreparedSSP(grid["flux"], grid["ax_met"], grid["ax_age"], grid["wave"], geo["z"],
+                          grid["log_axes"], dev)
+    tg = ops.TargetGrid(geo["wave_target"], dev)
+    gb = GalaxyBatch([w.stars for w in ws], [w.geom["nb"] for w in ws],
+                     [w.geom["edges"] for w in ws], dev)
+    L = gb.layout
+    sp = torch.tensor(geo["psf_sigma"], device=dev, requires_grad=True)
+    sl = torch.tensor(geo["lsf_sigma"], device=dev, requires_grad=True)
+    state = {"tune": ops.Tuning()}
+
+    def rend():
+        return gb.render(ssp, tg, sp, geo["psf_size"], sl, geo["wave_res"], tune=state["tune"])
+
+    with torch.no_grad():
+        tgt = rend() * 1.05
+        wt = _chi2_weights(tgt)
+    inputs = gb.leaves + [sp, sl]
+
+    def eager_step(tune):
+        state["tune"] = tune
+        cube = rend()
+        cot = 2.0 * (cube.detach() - tgt) * wt
+        torch.autograd.grad(cube, inputs, grad_outputs=cot)
+
+    key = autotune.problem_key(L.n_total, L.n_rows, nt, n_src, dev)
+    t0 = time.perf_counter()
+    best, seen = autotune.autotune(eager_step, key, reps=1, log=log)
+    log(f"autotune {time.perf_counter() - t0:.1f}s -> {best}")
+    state["tune"] = best
+    eager_ms = autotune.time_ms(eager_step, best, reps=3)
+    default_ms = autotune.time_ms(eager_step, ops.Tuning(), reps=3)
+
+    step = GraphedStep(rend, inputs, tgt, wt)
+    for _ in range(args.warmup):
+        step.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step.replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    work = L.n_total * nt
+
+    def loop(fn):
+        """One pass over the galaxies, each its own forward + adjoint."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for g, w in enumerate(ws):
+            s, r = L.stars_of(g), L.rows_of(g)
+            nb = w.geom["nb"]
+            leaf = {k: gb.t[k][s].detach().requires_grad_(True)
+                    for k in ("mass", "age", "metallicity", "velocity")}
+            out = fn(gb.t["coords"][s], leaf["velocity"], leaf["mass"], leaf["age"],
+                     leaf["metallicity"], gb.edges[g], ssp, tg, nb, sp, geo["psf_size"], sl,
+                     geo["wave_res"])
+            cot = 2.0 * (out.detach().reshape(nb * nb, nt) - tgt[r]) * wt[r]
+            torch.autograd.grad(out, list(leaf.values()) + [sp, sl],
+                                grad_outputs=cot.reshape(out.shape))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    loop(render)  # warm-up of allocations
+    fused_loop_ms = loop(render)
+    plain_loop_ms = loop(render_plain) if args.plain_steps > 0 else None
+    res = {
+        "metric": "star·λ-bins/s fwd+adjoint, 64 ragged galaxies in one pass (HIP graph)",
+        "value": work / (elapsed / args.steps), "unit": "star·λ-bins/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{L.n_gal} galaxies, {L.n_total} stars, {L.n_rows} cube rows x {nt}, "
+                               f"SSP 6x20x{n_src}, PSF 11x11, LSF sigma 1.2 A; chi^2 forward + adjoint",
+                   "n_galaxies": L.n_gal, "n_stars": L.n_total, "n_rows": L.n_rows,
+                   "n_target": nt, "n_source": n_src},
+        "tuning": {"chunk": best.chunk, "waves": best.waves, "persist": best.persist,
+                   "eager_ms": round(eager_ms, 3), "default_tuning_eager_ms": round(default_ms, 3),
+                   "candidates_ms": {k: round(v, 3) for k, v in seen.items()}},
+        "loop_fused_single_galaxy_ms": round(fused_loop_ms, 3),
+        "loop_plain_per_galaxy_ms": None if plain_loop_ms is None else round(plain_loop_ms, 3),
+        "speedup_vs_plain_loop": None if plain_loop_ms is None else round(plain_loop_ms / ms, 3),
+        "speedup_vs_fused_loop": round(fused_loop_ms / ms, 3),
     }
     print(json.dumps(res), flush=True)
 

@@ -70,12 +70,14 @@ int rbx_spaxel_sort(const int32_t* ids, int64_t n, int32_t n_keys,
  * hold log10 values and queries are log10'd before interpolation.
  * rbx_ssp_prepare fills cum (double, n_met*n_age*n_wave: per-template
  * prefix sums of flux[k]*(wave_obs[k]-wave_obs[k-1]), 0 at k = 0) and the
- * knot locator, n_locate int32 pairs.  With h = 1/locate_inv_h, entry m >= 1
- * holds k = (number of wave_obs <= locate_x0 + (m-1) h) - 1 and the bits of
- * wave_obs[k+1] as float (+inf past the last knot); entry 0 is {-1,
- * wave_obs[0]} (queries below locate_x0).  When h is at most the smallest
- * knot spacing every cell holds at most one knot and locate_multi = 0;
- * otherwise locate_multi = 1 and the kernels walk from the entry.
+ * knot locator, n_locate entries of 4 int32.  With h = 1/locate_inv_h, entry
+ * m >= 1 describes the cell starting at locate_x0 + (m-1) h and entry 0 every
+ * query below locate_x0: {k, bits of wave_obs[b], wave_obs[b+1],
+ * wave_obs[b+2] as float} with k = (number of wave_obs <= cell start) - 1
+ * (-1 for entry 0) and b = min(max(k, -1), n_wave-2); knots outside
+ * [0, n_wave) read as -inf / +inf.  When h is at most the smallest knot
+ * spacing every cell holds at most one knot and locate_multi = 0; otherwise
+ * locate_multi = 1 and the kernels step further from the entry.
  */
 typedef struct rbx_ssp {
   const float* flux;
@@ -95,14 +97,17 @@ int rbx_ssp_prepare(const rbx_ssp* ssp, double* cum_out, int32_t* locate_out,
 
 /* Telescope wavelength axis (rubix/core/ifu.py:254-255).  dwave is
  * calculate_diff(wave) (first element 0, rubix/spectra/ifu.py:83-102);
- * wmin/wmax are min/max of wave; ascending != 0 when wave is non-decreasing
- * (enables the windowed knot search; any order is accepted). */
+ * pairs holds (wave[j], dwave[j]) interleaved (2n floats, read by the fused
+ * kernels); wmin/wmax are min/max of wave; ascending != 0 when wave is
+ * non-decreasing (required by rbx_resample_bwd; the fused path accepts any
+ * order). */
 typedef struct rbx_target {
   const float* wave;
   const float* dwave;
   int32_t n;
   float wmin, wmax;
   int32_t ascending;
+  const float* pairs;
 } rbx_target;
 
 /* Per-star inputs of the fused path (pointers to (n,) arrays unless noted).
@@ -151,6 +156,66 @@ int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target,
                  const rbx_stars* stars, const float* dcube, float* dmass,
                  float* dage, float* dmet, float* dvlos, void* workspace,
                  size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------- batches
+ * Many galaxies with uneven star counts and spaxel grids in one pass (the
+ * reference renders one galaxy per pipeline run, rubix/core/pipeline.py;
+ * this has no counterpart there).  The stars of all galaxies are
+ * concatenated in one rbx_stars (n = total stars, ids = each galaxy's own
+ * spaxel ids from rbx_spaxel_assign, n_spaxel = total cube rows; n_keys is
+ * not used).  galaxies[g] (a device array of n_gal entries, star_begin and
+ * spaxel_begin ascending) owns stars [star_begin, next star_begin or n)
+ * and rows [spaxel_begin, spaxel_begin + nb*nb) of the batch cube
+ * (n_spaxel rows x target.n); its ids outside [0, nb*nb) are dropped like
+ * segment_sum drops them.  Each fused stage is one launch over the whole
+ * batch.  rbx_batch_bwd reads the per-star state rbx_batch_fwd left in the
+ * workspace and may use a different tuning. */
+typedef struct rbx_galaxy {
+  int64_t star_begin;
+  int32_t spaxel_begin;
+  int32_t nb;
+} rbx_galaxy;
+
+/* Launch tuning of the fused kernels (NULL = defaults): chunk = stars per
+ * reduction chunk of the cube accumulation (16..65536, 0 = 256); waves =
+ * wavefronts per workgroup of the per-star kernels (1, 2, 4, 8; 0 = 4);
+ * persist = 0 for one workgroup per work item, k > 0 for persistent grids of
+ * k workgroups per compute unit. */
+typedef struct rbx_tuning {
+  int32_t chunk;
+  int32_t waves;
+  int32_t persist;
+  int32_t reserved;
+} rbx_tuning;
+
+size_t rbx_batch_workspace_size(int64_t n, int32_t n_spaxel, int32_t n_target,
+                                const rbx_tuning* tune);
+int rbx_batch_fwd(const rbx_ssp* ssp, const rbx_target* target,
+                  const rbx_stars* stars, const rbx_galaxy* galaxies,
+                  int32_t n_gal, const rbx_tuning* tune, float* cube,
+                  void* workspace, size_t workspace_bytes, void* stream);
+int rbx_batch_bwd(const rbx_ssp* ssp, const rbx_target* target,
+                  const rbx_stars* stars, const rbx_galaxy* galaxies,
+                  int32_t n_gal, const rbx_tuning* tune, const float* dcube,
+                  float* dmass, float* dage, float* dmet, float* dvlos,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* One pass of the separable PSF over every galaxy of a batch cube
+ * (n_rows = total rows, each row n_target long; galaxy g's rows are its
+ * (nb, nb) spaxel grid, cube[y, x, l] layout as rubix/core/psf.py:56-57
+ * convolves it): axis 1 convolves along x, axis 0 along y,
+ * out[.., u, ..] = sum_t taps[t] in[.., u + start - t, ..] with zero padding
+ * inside each galaxy.  rbx_conv_batch_dot returns
+ * out[0] = sum a * conv(b) (double taps, double accumulation, fixed order);
+ * its workspace is rbx_dot_workspace_size(n_rows * n_target). */
+int rbx_conv_batch(const float* in, float* out, const rbx_galaxy* galaxies,
+                   int32_t n_gal, int32_t n_rows, int32_t n_target, int32_t axis,
+                   const float* taps, int32_t n_taps, int32_t start, void* stream);
+int rbx_conv_batch_dot(const float* a, const float* b, const rbx_galaxy* galaxies,
+                       int32_t n_gal, int32_t n_rows, int32_t n_target, int32_t axis,
+                       const double* taps, int32_t n_taps, int32_t start,
+                       double* out, void* workspace, size_t workspace_bytes,
+                       void* stream);
 
 /* ---------------------------------------------------------------- per step
  * One kernel per rubix.core step, materialising the same per-star arrays the

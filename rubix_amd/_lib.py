@@ -27,7 +27,8 @@ class RbxSsp(ctypes.Structure):
 
 class RbxTarget(ctypes.Structure):
     _fields_ = [("wave", c_void_p), ("dwave", c_void_p), ("n", c_int32),
-                ("wmin", c_float), ("wmax", c_float), ("ascending", c_int32)]
+                ("wmin", c_float), ("wmax", c_float), ("ascending", c_int32),
+                ("pairs", c_void_p)]
 
 
 class RbxStars(ctypes.Structure):
@@ -37,6 +38,15 @@ class RbxStars(ctypes.Structure):
         ("n_spaxel", c_int32), ("n_keys", c_int32), ("reserved", c_int32),
         ("ids", c_void_p), ("speed_of_light", c_float), ("reserved_f", c_float),
     ]
+
+
+class RbxGalaxy(ctypes.Structure):
+    _fields_ = [("star_begin", c_int64), ("spaxel_begin", c_int32), ("nb", c_int32)]
+
+
+class RbxTuning(ctypes.Structure):
+    _fields_ = [("chunk", c_int32), ("waves", c_int32), ("persist", c_int32),
+                ("reserved", c_int32)]
 
 
 # name -> (restype, argtypes); mirrors include/rubix_amd.h
@@ -53,6 +63,18 @@ SIGNATURES = {
                              c_void_p, c_size_t, c_void_p]),
     "rbx_cube_bwd": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars), c_void_p,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "rbx_batch_workspace_size": (c_size_t, [c_int64, c_int32, c_int32, POINTER(RbxTuning)]),
+    "rbx_batch_fwd": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars), c_void_p,
+                              c_int32, POINTER(RbxTuning), c_void_p, c_void_p, c_size_t,
+                              c_void_p]),
+    "rbx_batch_bwd": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars), c_void_p,
+                              c_int32, POINTER(RbxTuning), c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_size_t, c_void_p]),
+    "rbx_conv_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
+                               c_void_p, c_int32, c_int32, c_void_p]),
+    "rbx_conv_batch_dot": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                   c_int32, c_void_p, c_int32, c_int32, c_void_p, c_void_p,
+                                   c_size_t, c_void_p]),
     "rbx_ssp_lookup": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "rbx_ssp_lookup_bwd": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),

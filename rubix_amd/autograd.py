@@ -10,6 +10,8 @@ adjoint.
 * GaussianLSF     - 25-tap Gaussian LSF along lambda (sigma in Angstrom,
                     rubix/telescope/lsf/lsf.py:22-26); adjoint w.r.t. the cube
                     and sigma.
+* BatchedDatacube, BatchedPSF - the same for a ragged batch of galaxies in
+                    one pass (render_batched).
 
 The reference has no adjoint (rubix/core/pipeline.py:203-208) and keeps the
 PSF/LSF sigmas as python floats captured by the step closures
@@ -26,20 +28,29 @@ from . import ops
 
 # ---------------------------------------------------------------- taps
 
+def psf_x(size: int):
+    """Tap positions of gaussian_kernel_2d (rubix/telescope/psf/kernels.py:26-30):
+    arange(-(m-1)/2, (m-1)/2 + 1)."""
+    return np.arange(-((size - 1) / 2), ((size - 1) / 2) + 1, dtype=np.float64)
+
+
+def lsf_x(wave_res: float, factor: int = 12):
+    """Tap positions of the LSF kernel (rubix/telescope/lsf/lsf.py:22-26):
+    arange(-f dl, f dl + dl, dl)."""
+    return np.arange(-factor * wave_res, factor * wave_res + wave_res, wave_res, dtype=np.float64)
+
+
 def gaussian_taps_2d_factor(size: int, sigma: float):
     """1D factor g of gaussian_kernel_2d(size, size, sigma)
     (rubix/telescope/psf/kernels.py:5-31): K = outer(g, g) with
-    g = exp(-x^2 / 2 sigma^2) / sum, x = arange(-(m-1)/2, (m-1)/2 + 1).
-    Returns (g, dg/dsigma) in float64 numpy."""
-    x = np.arange(-((size - 1) / 2), ((size - 1) / 2) + 1, dtype=np.float64)
-    return _norm_gauss(x, float(sigma))
+    g = exp(-x^2 / 2 sigma^2) / sum.  Returns (g, dg/dsigma) in float64 numpy."""
+    return _norm_gauss(psf_x(size), float(sigma))
 
 
 def lsf_taps(sigma: float, wave_res: float, factor: int = 12):
     """LSF kernel of rubix/telescope/lsf/lsf.py:22-26 and its sigma-derivative:
-    x = arange(-f dl, f dl + dl, dl), g = exp(-x^2 / 2 sigma^2) / sum."""
-    x = np.arange(-factor * wave_res, factor * wave_res + wave_res, wave_res, dtype=np.float64)
-    return _norm_gauss(x, float(sigma))
+    g = exp(-x^2 / 2 sigma^2) / sum."""
+    return _norm_gauss(lsf_x(wave_res, factor), float(sigma))
 
 
 def _norm_gauss(x, sigma):
@@ -48,6 +59,37 @@ def _norm_gauss(x, sigma):
     x2 = x * x / sigma ** 3
     dg = g * (x2 - np.sum(g * x2))
     return g, dg
+
+
+_XGRID = {}
+
+
+def _xgrid(key, make, device):
+    """Device copy of a tap-position grid, made once per (grid, device), so a
+    step that is replayed from a HIP graph copies nothing from the host."""
+    k = (key, str(device))
+    t = _XGRID.get(k)
+    if t is None:
+        t = torch.as_tensor(make(), dtype=torch.float64, device=device)
+        _XGRID[k] = t
+    return t
+
+
+def _taps(key, make, sigma, device):
+    """(float32 taps, float64 d taps/d sigma) of the normalised Gaussian on the
+    grid `make()`.  A device-tensor sigma is evaluated on the device (no host
+    synchronisation); a python number on the host.  Both evaluate
+    g = exp(-x^2 / 2 sigma^2) / sum in float64 and round g once to float32."""
+    if isinstance(sigma, torch.Tensor) and sigma.is_cuda:
+        x = _xgrid(key, make, device)
+        s = sigma.detach().to(device=device, dtype=torch.float64).reshape(())
+        e = torch.exp(-0.5 * x * x / (s * s))
+        g = e / e.sum()
+        x2 = x * x / (s * s * s)
+        dg = g * (x2 - (g * x2).sum())
+        return g.to(torch.float32), dg
+    g, dg = _norm_gauss(make(), float(sigma))
+    return _dev(g, device), _dev64(dg, device)
 
 
 def _dev(a, device):
@@ -94,9 +136,7 @@ class GaussianPSF(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cube, sigma, size):
         ny, nx, nl = cube.shape
-        g, dg = gaussian_taps_2d_factor(size, float(sigma))
-        dev = cube.device
-        gt, dgt = _dev(g, dev), _dev64(dg, dev)
+        gt, dgt = _taps(("psf", size), lambda: psf_x(size), sigma, cube.device)
         start = (size - 1) // 2
         cube = cube.contiguous()
         tmp = ops.conv_axis(cube, gt, ny, nx, nl, start)             # along x
@@ -137,9 +177,8 @@ class GaussianLSF(torch.autograd.Function):
         shape = cube.shape
         nl = shape[-1]
         ns = cube.numel() // nl
-        k, dk = lsf_taps(float(sigma), float(wave_res), factor)
-        dev = cube.device
-        kt, dkt = _dev(k, dev), _dev64(dk, dev)
+        kt, dkt = _taps(("lsf", float(wave_res), factor), lambda: lsf_x(wave_res, factor), sigma,
+                        cube.device)
         cube = cube.contiguous()
         out = ops.conv_axis(cube, kt, ns, nl, 1, factor)
         ctx.save_for_backward(cube)
@@ -162,6 +201,72 @@ class GaussianLSF(torch.autograd.Function):
         dsig = None
         if ctx.needs_input_grad[1]:
             dsig = ops.conv_axis_dot(gout, cube, dkt, ns, nl, 1, ctx.factor).to(torch.float32)
+        return gin, dsig, None, None
+
+
+# ---------------------------------------------------------------- batches
+
+class BatchedDatacube(torch.autograd.Function):
+    """Every galaxy of a ragged batch in one fused pass: concatenated
+    (mass, age, metallicity, velocity) and per-galaxy spaxel ids -> batch
+    cube (n_rows, n_target)."""
+
+    @staticmethod
+    def forward(ctx, mass, age, met, velocity, ids, ssp, target, layout, tune, vel_comp, c):
+        cube, cctx = ops.batch_forward(ssp, target, layout, mass.detach(), age.detach(),
+                                       met.detach(), velocity.detach(), ids, tune, vel_comp, c)
+        ctx.cctx = cctx
+        ctx.args = (ssp, target, layout, tune, vel_comp)
+        ctx.vshape = velocity.shape
+        return cube
+
+    @staticmethod
+    def backward(ctx, dcube):
+        ssp, target, layout, tune, vel_comp = ctx.args
+        need = ctx.needs_input_grad
+        dm, dt, dz, dv = ops.batch_backward(ssp, target, layout, ctx.cctx, dcube.contiguous(),
+                                            need=(need[0], need[1], need[2], need[3]), tune=tune)
+        dvel = None
+        if need[3]:
+            dvel = torch.zeros(ctx.vshape, dtype=torch.float32, device=dcube.device)
+            dvel[:, vel_comp] = dv
+        return dm, dt, dz, dvel, None, None, None, None, None, None, None
+
+
+class BatchedPSF(torch.autograd.Function):
+    """Separable Gaussian PSF over every galaxy of a batch cube (n_rows,
+    n_target), each on its own (nb, nb) grid."""
+
+    @staticmethod
+    def forward(ctx, cube, sigma, size, layout):
+        gt, dgt = _taps(("psf", size), lambda: psf_x(size), sigma, cube.device)
+        start = (size - 1) // 2
+        cube = cube.contiguous()
+        tmp = ops.conv_batch(cube, layout, 1, gt, start)   # along x
+        out = ops.conv_batch(tmp, layout, 0, gt, start)    # along y
+        ctx.save_for_backward(cube, tmp)
+        ctx.taps = (gt, dgt)
+        ctx.size = size
+        ctx.layout = layout
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        cube, tmp = ctx.saved_tensors
+        gt, dgt = ctx.taps
+        m = ctx.size
+        layout = ctx.layout
+        start = (m - 1) // 2
+        rstart = m - 1 - start
+        grev = torch.flip(gt, (0,)).contiguous()
+        gout = gout.contiguous()
+        v = ops.conv_batch(gout, layout, 0, grev, rstart)
+        gin = ops.conv_batch(v, layout, 1, grev, rstart) if ctx.needs_input_grad[0] else None
+        dsig = None
+        if ctx.needs_input_grad[1]:
+            d1 = ops.conv_batch_dot(gout, tmp, layout, 0, dgt, start)
+            d2 = ops.conv_batch_dot(v, cube, layout, 1, dgt, start)
+            dsig = (d1 + d2).to(torch.float32)
         return gin, dsig, None, None
 
 
@@ -273,3 +378,17 @@ def render(coords, velocity, mass, age, met, edges, ssp, target, nb, psf_sigma, 
     if return_all:
         return ids, raw, c1, c2
     return c2
+
+
+def render_batched(coords, velocity, mass, age, met, edges, layout, ssp, target, psf_sigma,
+                   psf_size, lsf_sigma, wave_res, tune=None, vel_comp=2, c=ops.SPEED_OF_LIGHT):
+    """spaxel assignment (per galaxy, its own edges) -> one fused pass over
+    the whole batch -> PSF per galaxy -> LSF; differentiable like render().
+    Star arrays are the galaxies' concatenated arrays (layout.star_begin);
+    returns the batch cube (n_rows, n_target)."""
+    ids = torch.cat([ops.spaxel_assign(coords[layout.stars_of(g)], edges[g])
+                     for g in range(layout.n_gal)])
+    raw = BatchedDatacube.apply(mass, age, met, velocity, ids, ssp, target, layout, tune,
+                                vel_comp, c)
+    c1 = BatchedPSF.apply(raw, psf_sigma, psf_size, layout)
+    return GaussianLSF.apply(c1, lsf_sigma, wave_res)

@@ -107,6 +107,97 @@ __global__ void __launch_bounds__(kDotThreads) dot_final_kernel(const double* __
   if (threadIdx.x == 0) *result = sm[0];
 }
 
+// ------------------------------------------------------------ batches
+//
+// A batch cube stacks the (nb, nb, n_lambda) cubes of several galaxies row
+// after row (rbx_galaxy: first row and nb of each).  One launch convolves
+// every galaxy: each output element finds its galaxy by binary search over
+// the (ascending) first rows, then runs the same 1D convolution inside that
+// galaxy's grid with zero padding at its edges.
+
+struct BatchPos {
+  bool ok;      // the row belongs to a galaxy whose grid fits the batch
+  int64_t base; // element of coordinate u = 0 along the convolved axis
+  int64_t stride;
+  int u, nb;
+};
+
+__device__ inline BatchPos batch_pos(const rbx_galaxy* __restrict__ gal, int n_gal, int n_rows,
+                                     int nt, int axis, int64_t idx) {
+  BatchPos b;
+  const int p = int(idx / nt);
+  const int l = int(idx - int64_t(p) * nt);
+  int lo = 0, hi = n_gal;  // last galaxy with spaxel_begin <= p
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (gal[mid].spaxel_begin <= p) lo = mid; else hi = mid;
+  }
+  const int64_t begin = gal[lo].spaxel_begin;
+  const int nb = gal[lo].nb;
+  const int64_t q = p - begin;
+  b.nb = nb;
+  b.ok = nb > 0 && q >= 0 && q < int64_t(nb) * nb && begin + int64_t(nb) * nb <= n_rows;
+  if (!b.ok) {
+    b.base = 0; b.stride = 0; b.u = 0;
+    return b;
+  }
+  const int y = int(q / nb), x = int(q - int64_t(y) * nb);
+  if (axis == 1) {
+    b.u = x;
+    b.stride = nt;
+    b.base = (begin + int64_t(y) * nb) * nt + l;
+  } else {
+    b.u = y;
+    b.stride = int64_t(nb) * nt;
+    b.base = (begin + x) * nt + l;
+  }
+  return b;
+}
+
+__global__ void conv_batch_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                  const rbx_galaxy* __restrict__ gal, int n_gal, int n_rows,
+                                  int nt, int axis, const float* __restrict__ taps, int n_taps,
+                                  int start) {
+  const int64_t total = int64_t(n_rows) * nt;
+  for (int64_t idx = gtid(); idx < total; idx += int64_t(gridDim.x) * blockDim.x) {
+    const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, idx);
+    float acc = 0.f;
+    if (b.ok) {
+      for (int t = 0; t < n_taps; ++t) {
+        const int us = b.u + start - t;
+        if (us >= 0 && us < b.nb) acc += taps[t] * in[b.base + us * b.stride];
+      }
+    }
+    out[idx] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(kDotThreads) conv_batch_dot_partial_kernel(
+    const float* __restrict__ a, const float* __restrict__ bsrc,
+    const rbx_galaxy* __restrict__ gal, int n_gal, int n_rows, int nt, int axis,
+    const double* __restrict__ taps, int n_taps, int start, double* __restrict__ partial) {
+  __shared__ double sm[kDotThreads];
+  const int64_t total = int64_t(n_rows) * nt;
+  double acc = 0.0;
+  for (int64_t idx = gtid(); idx < total; idx += int64_t(gridDim.x) * blockDim.x) {
+    const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, idx);
+    if (!b.ok) continue;
+    double c = 0.0;
+    for (int t = 0; t < n_taps; ++t) {
+      const int us = b.u + start - t;
+      if (us >= 0 && us < b.nb) c += taps[t] * double(bsrc[b.base + us * b.stride]);
+    }
+    acc += double(a[idx]) * c;
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kDotThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
 static int grid_for(int64_t n, int threads, int64_t cap) {
   int64_t g = (n + threads - 1) / threads;
   if (g < 1) g = 1;
@@ -185,6 +276,42 @@ int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer, int32_t n
                      n_axis, n_inner, taps, n_taps, start, partial);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, nblk, result);
   return check_launch("rbx_conv_axis_dot");
+}
+
+int rbx_conv_batch(const float* in, float* out, const rbx_galaxy* galaxies, int32_t n_gal,
+                   int32_t n_rows, int32_t n_target, int32_t axis, const float* taps,
+                   int32_t n_taps, int32_t start, void* stream) {
+  using namespace rbx;
+  if (!in || !out || !galaxies || !taps || n_gal < 1 || n_rows < 0 || n_target < 1 ||
+      n_taps < 1 || (axis != 0 && axis != 1))
+    return fail(RBX_EINVAL, "rbx_conv_batch: bad arguments");
+  if (in == out) return fail(RBX_EINVAL, "rbx_conv_batch: in-place convolution not supported");
+  const int64_t total = int64_t(n_rows) * n_target;
+  if (total == 0) return RBX_OK;
+  hipLaunchKernelGGL(conv_batch_kernel, dim3(grid_for(total, 256, 1 << 20)), dim3(256), 0,
+                     as_stream(stream), in, out, galaxies, n_gal, n_rows, n_target, axis, taps,
+                     n_taps, start);
+  return check_launch("rbx_conv_batch");
+}
+
+int rbx_conv_batch_dot(const float* a, const float* b, const rbx_galaxy* galaxies, int32_t n_gal,
+                       int32_t n_rows, int32_t n_target, int32_t axis, const double* taps,
+                       int32_t n_taps, int32_t start, double* out, void* workspace,
+                       size_t workspace_bytes, void* stream) {
+  using namespace rbx;
+  if (!a || !b || !galaxies || !taps || !out || n_gal < 1 || n_rows < 0 || n_target < 1 ||
+      n_taps < 1 || (axis != 0 && axis != 1))
+    return fail(RBX_EINVAL, "rbx_conv_batch_dot: bad arguments");
+  const int64_t total = int64_t(n_rows) * n_target;
+  const int nblk = dot_blocks(total);
+  if (!workspace || workspace_bytes < size_t(nblk) * sizeof(double))
+    return fail(RBX_EWORKSPACE, "rbx_conv_batch_dot: workspace too small");
+  hipStream_t s = as_stream(stream);
+  double* partial = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(conv_batch_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b,
+                     galaxies, n_gal, n_rows, n_target, axis, taps, n_taps, start, partial);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, nblk, out);
+  return check_launch("rbx_conv_batch_dot");
 }
 
 }  // extern "C"

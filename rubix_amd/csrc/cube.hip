@@ -37,6 +37,7 @@
 
 #include "rbx_common.h"
 #include "ssp_device.h"
+#include "cube_impl.h"
 
 namespace rbx {
 
@@ -57,7 +58,7 @@ struct __align__(16) StarState {
 static_assert(sizeof(StarState) == 64, "StarState must be 64 bytes");
 // The C-ABI descriptors are mirrored by ctypes structures; pin their layout.
 static_assert(sizeof(rbx_ssp) == 80, "rbx_ssp layout");
-static_assert(sizeof(rbx_target) == 32, "rbx_target layout");
+static_assert(sizeof(rbx_target) == 40, "rbx_target layout");
 static_assert(sizeof(rbx_stars) == 80, "rbx_stars layout");
 
 // sum_{k=kl..kh} f_c[k] (wave[k] - wave[k-1]) for each corner, in double.
@@ -179,11 +180,17 @@ __device__ inline Knots knots_of(const SspView& g, float x, const Shift& h) {
   const int n = g.n_wave;
   float fm = fmaf(x, h.c1, h.c0);
   fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
-  const int2 e = ld_at(reinterpret_cast<const int2*>(g.loc), uint32_t(fm));
+  const int4 e = ld_at(reinterpret_cast<const int4*>(g.loc), uint32_t(fm));
   const float y = x * h.inv_a;
-  const int k = e.x + (__int_as_float(e.y) <= y ? 1 : 0);
+  const float e1 = __int_as_float(e.y), e2 = __int_as_float(e.z), e3 = __int_as_float(e.w);
+  const int k = e.x + (e2 <= y ? 1 : 0);  // e2 = the knot after k0 (see ssp_locate_kernel)
   int i = k + 1 < 1 ? 1 : (k + 1 > n - 1 ? n - 1 : k + 1);
-  Pair lp = ld_pair(g.wave, uint32_t(i - 1));
+  // the segment's knots from the entry: i - 1 - b is 0 or 1
+  const int b = e.x < n - 2 ? e.x : n - 2;
+  const bool hi = (i - 1 - b) > 0;
+  Pair lp;
+  lp.lo = hi ? e2 : e1;
+  lp.hi = hi ? e3 : e2;
   float wl = mul_nocontract(lp.lo, h.a), wr = mul_nocontract(lp.hi, h.a);
   // exact segment: i in [1, n-1] moves monotonically, so the loop ends
   while ((x >= wr && i < n - 1) || (x < wl && i > 1)) {
@@ -235,49 +242,56 @@ __device__ inline float star_value(const Rows& R, const StarState& s, const Knot
   return S0 + K.theta * (S1 - S0);
 }
 
-// Wave and star of this thread in the slab-mapped kernels: workgroup b runs
-// on XCD b % 8 and handles lambda slab b % 8 of four stars, one per
-// wavefront, so each XCD's L2 holds only its slab of the templates.
+// Work items of the slab-mapped per-star kernels: item q is lambda slab
+// q % 8 of the W stars [(q / 8) W, (q / 8) W + W), one per wavefront of a
+// workgroup of W waves.  Workgroup b takes items b, b + G, b + 2G, ... with
+// G = gridDim.x a multiple of 8 (G = number of items, or a persistent grid
+// of a few workgroups per CU), so it always holds slab b % 8 and runs on
+// XCD b % 8: each XCD's L2 holds only its slab of the templates.
 struct SlabTask {
   int slab, wave, lane;
   int64_t r;
 };
 
-__device__ inline SlabTask slab_task() {
+template <int W>
+__device__ inline SlabTask slab_task(int64_t q) {
   SlabTask t;
-  t.slab = blockIdx.x & (kSlabs - 1);
+  t.slab = int(q & (kSlabs - 1));
   t.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   t.lane = threadIdx.x & 63;
-  t.r = int64_t(blockIdx.x >> 3) * 4 + t.wave;
+  t.r = (q >> 3) * W + t.wave;
   return t;
 }
 
 // ------------------------------------------------------------ norm
 
 // Partial Nn of (star, slab); star_sf_kernel adds the slabs in a fixed order.
-__global__ void __launch_bounds__(256) star_norm_kernel(SspView g, rbx_target tg, int64_t n,
-                                                        const StarState* __restrict__ state,
-                                                        int slab_bins,
-                                                        float* __restrict__ nn_part) {
-  const SlabTask t = slab_task();
-  if (t.r >= n) return;
-  const StarState s = state[t.r];
-  if (s.row < 0) {
-    if (t.lane == 0) nn_part[t.r * kSlabs + t.slab] = 0.f;
-    return;
-  }
-  const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
-  const Shift h = shift_of(g, s.a, s.inv_a);
-  const uint32_t j0 = uint32_t(t.slab * slab_bins);
-  const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
-  float acc = 0.f;
+template <int W>
+__global__ void __launch_bounds__(64 * W) star_norm_kernel(SspView g, rbx_target tg, int64_t n,
+                                                           int64_t n_items,
+                                                           const StarState* __restrict__ state,
+                                                           int slab_bins,
+                                                           float* __restrict__ nn_part) {
+  for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
+    const SlabTask t = slab_task<W>(q);
+    if (t.r >= n) continue;
+    const StarState s = state[t.r];
+    float acc = 0.f;
+    if (s.row >= 0) {
+      const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
+      const Shift h = shift_of(g, s.a, s.inv_a);
+      const uint32_t j0 = uint32_t(t.slab * slab_bins);
+      const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
 #pragma unroll 2
-  for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
-    const Knots K = knots_of<false>(g, ld_at(tg.wave, j), h);
-    acc += star_value(R, s, K) * ld_at(tg.dwave, j);
+      for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
+        const Pair xd = ld_pair(tg.pairs, 2 * j);
+        const Knots K = knots_of<false>(g, xd.lo, h);
+        acc += star_value(R, s, K) * xd.hi;
+      }
+      acc = wave_sum(acc);
+    }
+    if (t.lane == 0) nn_part[t.r * kSlabs + t.slab] = acc;
   }
-  acc = wave_sum(acc);
-  if (t.lane == 0) nn_part[t.r * kSlabs + t.slab] = acc;
 }
 
 __global__ void star_sf_kernel(int64_t n, const float* __restrict__ nn_part,
@@ -297,28 +311,28 @@ __global__ void star_sf_kernel(int64_t n, const float* __restrict__ nn_part,
 // ------------------------------------------------------------ chunks
 
 __global__ void chunk_count_kernel(const int32_t* __restrict__ offsets, int32_t n_spaxel,
-                                   int32_t* __restrict__ chunk_off) {
+                                   int chunk, int32_t* __restrict__ chunk_off) {
   for (int64_t p = gtid(); p <= n_spaxel; p += int64_t(gridDim.x) * blockDim.x) {
     if (p == n_spaxel) {
       chunk_off[p] = 0;
     } else {
       const int32_t cnt = offsets[p + 1] - offsets[p];
-      chunk_off[p] = (cnt + kChunk - 1) / kChunk;
+      chunk_off[p] = (cnt + chunk - 1) / chunk;
     }
   }
 }
 
 __global__ void chunk_fill_kernel(const int32_t* __restrict__ offsets, int32_t n_spaxel,
-                                  const int32_t* __restrict__ chunk_off,
+                                  int chunk, const int32_t* __restrict__ chunk_off,
                                   ChunkDesc* __restrict__ desc) {
   for (int64_t p = gtid(); p < n_spaxel; p += int64_t(gridDim.x) * blockDim.x) {
     const int32_t b = offsets[p], e = offsets[p + 1];
     int32_t o = chunk_off[p];
-    for (int32_t s = b; s < e; s += kChunk, ++o) {
+    for (int32_t s = b; s < e; s += chunk, ++o) {
       ChunkDesc d;
       d.spaxel = int32_t(p);
       d.begin = s;
-      d.end = s + kChunk < e ? s + kChunk : e;
+      d.end = s + chunk < e ? s + chunk : e;
       d.pad = 0;
       desc[o] = d;
     }
@@ -327,33 +341,38 @@ __global__ void chunk_fill_kernel(const int32_t* __restrict__ offsets, int32_t n
 
 // ------------------------------------------------------------ accumulate
 
+// Item q: 64-bin lambda tile of XCD q % 8's slab and one spaxel chunk; the
+// items are strided over the grid like the per-star kernels' (G a multiple
+// of 8), so workgroup b stays on slab b % 8.
 __global__ void __launch_bounds__(64) cube_accum_kernel(
     SspView g, rbx_target tg, const StarState* __restrict__ state,
     const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr,
-    int n_tiles, int tiles_per_xcd, double* __restrict__ partial) {
-  const int b = blockIdx.x;
-  const int xcd = b & 7;
-  const int q = b >> 3;
-  const int tile = xcd * tiles_per_xcd + (q % tiles_per_xcd);
-  const int chunk = q / tiles_per_xcd;
-  if (tile >= n_tiles || chunk >= *n_chunks_ptr) return;
-  const ChunkDesc d = desc[chunk];
-  const int j = tile * kTile + int(threadIdx.x);
-  const bool valid = j < tg.n;
-  const float x = ld_at(tg.wave, uint32_t(valid ? j : tg.n - 1));
-  // double accumulator: the cube is a sum of up to chunk-size positive
-  // terms per lane; accumulating in double keeps the forward's rounding at
-  // the level of the individual float32 terms.
-  double acc = 0.0;
+    int n_tiles, int tiles_per_xcd, int64_t n_items, double* __restrict__ partial) {
+  const int n_chunks = *n_chunks_ptr;
+  for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
+    const int xcd = int(q & 7);
+    const int64_t w = q >> 3;
+    const int tile = xcd * tiles_per_xcd + int(w % tiles_per_xcd);
+    const int64_t chunk = w / tiles_per_xcd;
+    if (tile >= n_tiles || chunk >= n_chunks) continue;
+    const ChunkDesc d = desc[chunk];
+    const int j = tile * kTile + int(threadIdx.x);
+    const bool valid = j < tg.n;
+    const float x = ld_at(tg.wave, uint32_t(valid ? j : tg.n - 1));
+    // double accumulator: the cube is a sum of up to chunk-size positive
+    // terms per lane; accumulating in double keeps the forward's rounding
+    // at the level of the individual float32 terms.
+    double acc = 0.0;
 #pragma unroll 2
-  for (int r = d.begin; r < d.end; ++r) {
-    const StarState s = state[r];
-    if (s.row < 0) continue;
-    const Rows R = rows_of(g, s.row);
-    const Knots K = knots_of<false>(g, x, shift_of(g, s.a, s.inv_a));
-    acc += double(star_value(R, s, K) * s.sf);
+    for (int r = d.begin; r < d.end; ++r) {
+      const StarState s = state[r];
+      if (s.row < 0) continue;
+      const Rows R = rows_of(g, s.row);
+      const Knots K = knots_of<false>(g, x, shift_of(g, s.a, s.inv_a));
+      acc += double(star_value(R, s, K) * s.sf);
+    }
+    if (valid) partial[chunk * tg.n + j] = acc;
   }
-  if (valid) partial[int64_t(chunk) * tg.n + j] = acc;
 }
 
 __global__ void cube_combine_kernel(const double* __restrict__ partial,
@@ -395,13 +414,16 @@ __device__ inline double lds_sum16(const float v[16], float* buf, int lane) {
 }
 
 // Slab-mapped like star_norm_kernel; writes the 16 sums of (star, slab) to
-// vpart (value sums, float) / apart (Doppler sums, double).
-__global__ void __launch_bounds__(256) star_bwd_kernel(
-    SspView g, rbx_target tg, int64_t n, const StarState* __restrict__ state,
+// vpart (value sums, float) / apart (Doppler sums, double).  The item loop is
+// uniform over the workgroup (lds_sum16 synchronises it).
+template <int W>
+__global__ void __launch_bounds__(64 * W) star_bwd_kernel(
+    SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
     const float* __restrict__ dcube, int slab_bins, float* __restrict__ vpart,
     double* __restrict__ apart) {
-  __shared__ float red[4][64 * kRedStride];
-  const SlabTask t = slab_task();
+  __shared__ float red[W][64 * kRedStride];
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+  const SlabTask t = slab_task<W>(item);
   const bool live = t.r < n;
   float v[16];
 #pragma unroll
@@ -417,8 +439,9 @@ __global__ void __launch_bounds__(256) star_bwd_kernel(
     const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
 #pragma unroll 2
     for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
-      const float gj = ld_at(grow, j), dj = ld_at(tg.dwave, j);
-      const Knots K = knots_of<true>(g, ld_at(tg.wave, j), h);
+      const Pair xd = ld_pair(tg.pairs, 2 * j);
+      const float gj = ld_at(grow, j), dj = xd.hi;
+      const Knots K = knots_of<true>(g, xd.lo, h);
       const float gk = -gj * K.kappa, dk = -dj * K.kappa;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -439,6 +462,7 @@ __global__ void __launch_bounds__(256) star_bwd_kernel(
     const int64_t base = (t.r * kSlabs + t.slab) * 8;
     if (q < 8) vpart[base + q] = float(tot);
     else apart[base + q - 8] = tot;
+  }
   }
 }
 
@@ -539,8 +563,9 @@ __global__ void __launch_bounds__(256) ssp_cum_kernel(const float* __restrict__ 
 }
 
 // Locator entries (layout in the rbx_ssp comment of the C header): entry 0
-// = {-1, wave[0]}; entry m >= 1 = {k, wave[k + 1]} with k the last knot
-// <= x0 + (m - 1) h (double comparison), +inf after the last knot.
+// covers queries below x0, entry m >= 1 the cell starting at x0 + (m - 1) h;
+// k = last knot <= the cell start (double comparison), then the three knots
+// from b = min(max(k, -1), n - 2), -inf / +inf outside the grid.
 __global__ void ssp_locate_kernel(const float* __restrict__ wave, int nw, float x0, float inv_h,
                                   int nloc, int32_t* __restrict__ loc) {
   for (int64_t m = gtid(); m < nloc; m += int64_t(gridDim.x) * blockDim.x) {
@@ -554,13 +579,21 @@ __global__ void ssp_locate_kernel(const float* __restrict__ wave, int nw, float 
       }
       k = lo - 1;
     }
-    loc[2 * m] = k;
-    loc[2 * m + 1] = __float_as_int(k + 1 < nw ? wave[k + 1] : INFINITY);
+    const int b = k < nw - 2 ? k : nw - 2;
+    loc[4 * m] = k;
+    for (int q = 0; q < 3; ++q) {
+      const int kk = b + q;
+      const float v = kk < 0 ? -INFINITY : (kk >= nw ? INFINITY : wave[kk]);
+      loc[4 * m + 1 + q] = __float_as_int(v);
+    }
   }
 }
 
 // ------------------------------------------------------------ host side
 
+// Workspace layout.  The per-star arrays the adjoint reads (perm, skey,
+// state and the slab partials) come first and do not depend on the chunk
+// size, so the adjoint may run with any tuning.
 struct CubeLayout {
   int32_t *perm, *skey, *offsets, *chunk_off, *scan_blocks;
   StarState* state;
@@ -574,33 +607,46 @@ struct CubeLayout {
   int64_t max_chunks;
 };
 
+// per_star_only: carve just the per-star arrays (what the adjoint reads).
 template <class C>
-static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, CubeLayout* L) {
-  const int64_t max_chunks = (n + kChunk - 1) / kChunk + n_keys;
+static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, CubeLayout* L,
+                        bool per_star_only = false) {
+  const int64_t max_chunks = (n + chunk - 1) / chunk + n_keys;
   const size_t sort_bytes = sort_workspace_size(n, n_keys);
-  auto* perm = c.template take<int32_t>(n > 0 ? n : 1);
-  auto* skey = c.template take<int32_t>(n > 0 ? n : 1);
+  const size_t n1 = size_t(n > 0 ? n : 1);
+  auto* perm = c.template take<int32_t>(n1);
+  auto* skey = c.template take<int32_t>(n1);
+  auto* state = c.template take<StarState>(n1);
+  auto* nnp = c.template take<float>(n1 * kSlabs);
+  auto* vp = c.template take<float>(n1 * kSlabs * 8);
+  auto* ap = c.template take<double>(n1 * kSlabs * 8);
+  if (per_star_only) {
+    if (L) {
+      L->perm = reinterpret_cast<int32_t*>(perm);
+      L->skey = reinterpret_cast<int32_t*>(skey);
+      L->state = reinterpret_cast<StarState*>(state);
+      L->nn_part = reinterpret_cast<float*>(nnp);
+      L->vpart = reinterpret_cast<float*>(vp);
+      L->apart = reinterpret_cast<double*>(ap);
+    }
+    return;
+  }
   auto* offs = c.template take<int32_t>(size_t(n_keys) + 1);
   auto* coff = c.template take<int32_t>(size_t(n_keys) + 1);
   auto* blocks = c.template take<int32_t>((size_t(n_keys) + 1) / 4096 + 2);
-  auto* state = c.template take<StarState>(n > 0 ? n : 1);
-  auto* desc = c.template take<ChunkDesc>(max_chunks > 0 ? max_chunks : 1);
+  auto* desc = c.template take<ChunkDesc>(size_t(max_chunks > 0 ? max_chunks : 1));
   auto* part = c.template take<double>(size_t(max_chunks > 0 ? max_chunks : 1) * nt);
   auto* sws = c.template take<char>(sort_bytes);
-  const size_t ns = size_t(n > 0 ? n : 1) * kSlabs;
-  auto* nnp = c.template take<float>(ns);
-  auto* vp = c.template take<float>(ns * 8);
-  auto* ap = c.template take<double>(ns * 8);
   if (L) {
+    L->perm = reinterpret_cast<int32_t*>(perm);
+    L->skey = reinterpret_cast<int32_t*>(skey);
+    L->state = reinterpret_cast<StarState*>(state);
     L->nn_part = reinterpret_cast<float*>(nnp);
     L->vpart = reinterpret_cast<float*>(vp);
     L->apart = reinterpret_cast<double*>(ap);
-    L->perm = reinterpret_cast<int32_t*>(perm);
-    L->skey = reinterpret_cast<int32_t*>(skey);
     L->offsets = reinterpret_cast<int32_t*>(offs);
     L->chunk_off = reinterpret_cast<int32_t*>(coff);
     L->scan_blocks = reinterpret_cast<int32_t*>(blocks);
-    L->state = reinterpret_cast<StarState*>(state);
     L->desc = reinterpret_cast<ChunkDesc*>(desc);
     L->partial = reinterpret_cast<double*>(part);
     L->sort_ws = sws;
@@ -625,6 +671,71 @@ static int grid_for(int64_t n, int threads, int64_t cap = 1 << 20) {
   return int(gsz);
 }
 
+int tune_of(const rbx_tuning* t, Tune* out) {
+  Tune d;
+  d.chunk = 256;
+  d.waves = 4;
+  d.persist = 0;
+  if (t) {
+    if (t->chunk != 0) d.chunk = t->chunk;
+    if (t->waves != 0) d.waves = t->waves;
+    d.persist = t->persist;
+  }
+  if (d.chunk < 16 || d.chunk > 65536)
+    return fail(RBX_EINVAL, "rbx_tuning: chunk must be in [16, 65536] (0 = default)");
+  if (d.waves != 1 && d.waves != 2 && d.waves != 4 && d.waves != 8)
+    return fail(RBX_EINVAL, "rbx_tuning: waves must be 1, 2, 4 or 8 (0 = default)");
+  if (d.persist < 0 || d.persist > 64)
+    return fail(RBX_EINVAL, "rbx_tuning: persist must be in [0, 64]");
+  *out = d;
+  return RBX_OK;
+}
+
+// Compute units of the current device (cached per device).
+static int n_cus() {
+  static int cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cache[dev] <= 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v < 1)
+      v = 256;
+    cache[dev] = v;
+  }
+  return cache[dev];
+}
+
+// Workgroups for `items` slab-mapped work items (a multiple of 8): one per
+// item, or a persistent grid of `persist` workgroups per CU; always a
+// multiple of 8.  0 if the grid does not fit one launch.
+static int64_t item_grid(int64_t items, int persist) {
+  int64_t g = items;
+  if (persist > 0) {
+    const int64_t p = int64_t(persist) * n_cus();
+    if (p < g) g = p;
+  }
+  g = (g + 7) / 8 * 8;
+  if (g < 8) g = 8;
+  return g > INT32_MAX ? 0 : g;
+}
+
+template <int W>
+static void launch_norm(int64_t grid, hipStream_t s, const SspView& g, const rbx_target& tg,
+                        int64_t n, int64_t items, const StarState* st, int slab_bins,
+                        float* nn) {
+  hipLaunchKernelGGL(star_norm_kernel<W>, dim3(grid), dim3(64 * W), 0, s, g, tg, n, items, st,
+                     slab_bins, nn);
+}
+
+template <int W>
+static void launch_bwd(int64_t grid, hipStream_t s, const SspView& g, const rbx_target& tg,
+                       int64_t n, int64_t items, const StarState* st, const float* dcube,
+                       int slab_bins, float* vp, double* ap) {
+  hipLaunchKernelGGL(star_bwd_kernel<W>, dim3(grid), dim3(64 * W), 0, s, g, tg, n, items, st,
+                     dcube, slab_bins, vp, ap);
+}
+
 static int validate(const rbx_ssp* ssp, const rbx_target* tg, const rbx_stars* st) {
   if (!ssp || !tg || !st) return fail(RBX_EINVAL, "rbx_cube: null descriptor");
   if (ssp->n_met < 2 || ssp->n_age < 2 || ssp->n_wave < 2)
@@ -632,7 +743,8 @@ static int validate(const rbx_ssp* ssp, const rbx_target* tg, const rbx_stars* s
   if (!ssp->flux || !ssp->ax_met || !ssp->ax_age || !ssp->wave_obs || !ssp->cum || !ssp->locate ||
       ssp->n_locate < 1 || !(ssp->locate_inv_h > 0.f))
     return fail(RBX_EINVAL, "rbx_cube: SSP descriptor incomplete (run rbx_ssp_prepare)");
-  if (tg->n < 1 || !tg->wave || !tg->dwave) return fail(RBX_EINVAL, "rbx_cube: empty target grid");
+  if (tg->n < 1 || !tg->wave || !tg->dwave || !tg->pairs)
+    return fail(RBX_EINVAL, "rbx_cube: empty target grid (wave, dwave and pairs are required)");
   if (st->n < 0 || st->n > INT32_MAX - 1) return fail(RBX_EINVAL, "rbx_cube: bad star count");
   if (st->n > 0 && (!st->mass || !st->age || !st->metallicity || !st->velocity || !st->ids))
     return fail(RBX_EINVAL, "rbx_cube: null star array");
@@ -646,14 +758,14 @@ static int validate(const rbx_ssp* ssp, const rbx_target* tg, const rbx_stars* s
   return RBX_OK;
 }
 
-int build_chunks(const int32_t* offsets, int32_t ns, int32_t* chunk_off, int32_t* scan_blocks,
-                 ChunkDesc* desc, hipStream_t s) {
+int build_chunks(const int32_t* offsets, int32_t ns, int chunk, int32_t* chunk_off,
+                 int32_t* scan_blocks, ChunkDesc* desc, hipStream_t s) {
   hipLaunchKernelGGL(chunk_count_kernel, dim3(grid_for(int64_t(ns) + 1, 256)), dim3(256), 0, s,
-                     offsets, ns, chunk_off);
+                     offsets, ns, chunk, chunk_off);
   int rc = exclusive_scan_i32(chunk_off, int64_t(ns) + 1, scan_blocks, s);
   if (rc) return rc;
   hipLaunchKernelGGL(chunk_fill_kernel, dim3(grid_for(ns, 256)), dim3(256), 0, s, offsets, ns,
-                     chunk_off, desc);
+                     chunk, chunk_off, desc);
   return check_launch("build_chunks");
 }
 
@@ -661,6 +773,92 @@ void combine_chunks(const double* partial, const int32_t* chunk_off, int32_t ns,
                     float* cube, hipStream_t s) {
   hipLaunchKernelGGL(cube_combine_kernel, dim3(grid_for(int64_t(ns) * nt, 256)), dim3(256), 0, s,
                      partial, chunk_off, ns, nt, cube);
+}
+
+size_t cube_workspace_impl(int64_t n, int32_t n_keys, int32_t nt, const Tune& t) {
+  if (n < 0 || n_keys < 1 || nt < 1) return 0;
+  SizeOnly so;
+  cube_layout(so, n, n_keys, nt, t.chunk, nullptr);
+  return so.s.used;
+}
+
+int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
+                  const Tune& t, float* cube, void* workspace, size_t workspace_bytes,
+                  hipStream_t s, const char* who) {
+  int rc = validate(ssp, target, stars);
+  if (rc) return rc;
+  if (!cube) return failf(RBX_EINVAL, "%s: null cube", who);
+  const int64_t n = stars->n;
+  const int32_t nt = target->n;
+  Carver c(workspace, workspace_bytes);
+  CubeLayout L;
+  cube_layout(c, n, stars->n_keys, nt, t.chunk, &L);
+  if (!c.ok) return failf(RBX_EWORKSPACE, "%s: workspace too small", who);
+  const SspView g = view_of(*ssp);
+  rc = spaxel_sort_impl(stars->ids, n, stars->n_keys, L.perm, L.skey, L.offsets, L.sort_ws,
+                        L.sort_ws_bytes, s);
+  if (rc) return rc;
+  const int n_tiles = (nt + kTile - 1) / kTile;
+  const int tps = (n_tiles + kSlabs - 1) / kSlabs;
+  if (n > 0) {
+    hipLaunchKernelGGL(star_prep_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *target,
+                       *stars, L.perm, L.skey, L.state);
+    const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
+    const int64_t grid = item_grid(items, t.persist);
+    if (!grid) return failf(RBX_EINVAL, "%s: too many stars for one launch", who);
+    switch (t.waves) {
+      case 1: launch_norm<1>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part); break;
+      case 2: launch_norm<2>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part); break;
+      case 8: launch_norm<8>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part); break;
+      default: launch_norm<4>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part);
+    }
+    hipLaunchKernelGGL(star_sf_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L.nn_part,
+                       L.state);
+  }
+  const int32_t ns = stars->n_spaxel;
+  rc = build_chunks(L.offsets, ns, t.chunk, L.chunk_off, L.scan_blocks, L.desc, s);
+  if (rc) return rc;
+  const int tpx = (n_tiles + 7) / 8;
+  if (n > 0) {
+    const int64_t items = int64_t(8) * tpx * L.max_chunks;
+    const int64_t grid = item_grid(items, t.persist);
+    if (!grid) return failf(RBX_EINVAL, "%s: problem too large for one launch", who);
+    hipLaunchKernelGGL(cube_accum_kernel, dim3(grid), dim3(64), 0, s, g, *target, L.state,
+                       L.desc, L.chunk_off + ns, n_tiles, tpx, items, L.partial);
+  }
+  combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s);
+  return check_launch(who);
+}
+
+int cube_bwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
+                  const Tune& t, const float* dcube, float* dmass, float* dage, float* dmet,
+                  float* dvlos, void* workspace, size_t workspace_bytes, hipStream_t s,
+                  const char* who) {
+  int rc = validate(ssp, target, stars);
+  if (rc) return rc;
+  if (!dcube) return failf(RBX_EINVAL, "%s: null dcube", who);
+  const int64_t n = stars->n;
+  Carver c(workspace, workspace_bytes);
+  CubeLayout L;
+  cube_layout(c, n, stars->n_keys, target->n, t.chunk, &L, true);
+  if (!c.ok) return failf(RBX_EWORKSPACE, "%s: workspace too small", who);
+  if (n == 0) return RBX_OK;
+  const SspView g = view_of(*ssp);
+  const int32_t nt = target->n;
+  const int n_tiles = (nt + kTile - 1) / kTile;
+  const int tps = (n_tiles + kSlabs - 1) / kSlabs;
+  const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
+  const int64_t grid = item_grid(items, t.persist);
+  if (!grid) return failf(RBX_EINVAL, "%s: too many stars for one launch", who);
+  switch (t.waves) {
+    case 1: launch_bwd<1>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
+    case 2: launch_bwd<2>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
+    case 8: launch_bwd<8>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
+    default: launch_bwd<4>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart);
+  }
+  hipLaunchKernelGGL(star_grad_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *stars,
+                     L.state, L.vpart, L.apart, dmass, dage, dmet, dvlos);
+  return check_launch(who);
 }
 
 }  // namespace rbx
@@ -683,83 +881,26 @@ int rbx_ssp_prepare(const rbx_ssp* ssp, double* cum_out, int32_t* locate_out, vo
 }
 
 size_t rbx_cube_workspace_size(int64_t n, int32_t n_keys, int32_t n_target) {
-  using namespace rbx;
-  if (n < 0 || n_keys < 1 || n_target < 1) return 0;
-  SizeOnly so;
-  cube_layout(so, n, n_keys, n_target, nullptr);
-  return so.s.used;
+  rbx::Tune t;
+  rbx::tune_of(nullptr, &t);
+  return rbx::cube_workspace_impl(n, n_keys, n_target, t);
 }
 
 int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
                  float* cube, void* workspace, size_t workspace_bytes, void* stream) {
-  using namespace rbx;
-  int rc = validate(ssp, target, stars);
-  if (rc) return rc;
-  if (!cube) return fail(RBX_EINVAL, "rbx_cube_fwd: null cube");
-  hipStream_t s = as_stream(stream);
-  const int64_t n = stars->n;
-  const int32_t nt = target->n;
-  Carver c(workspace, workspace_bytes);
-  CubeLayout L;
-  cube_layout(c, n, stars->n_keys, nt, &L);
-  if (!c.ok) return fail(RBX_EWORKSPACE, "rbx_cube_fwd: workspace too small");
-  const SspView g = view_of(*ssp);
-  rc = spaxel_sort_impl(stars->ids, n, stars->n_keys, L.perm, L.skey, L.offsets, L.sort_ws,
-                        L.sort_ws_bytes, s);
-  if (rc) return rc;
-  if (n > 0) {
-    hipLaunchKernelGGL(star_prep_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *target,
-                       *stars, L.perm, L.skey, L.state);
-    const int n_tiles = (nt + kTile - 1) / kTile;
-    const int tps = (n_tiles + kSlabs - 1) / kSlabs;
-    const int64_t blocks = int64_t(kSlabs) * ((n + 3) / 4);
-    if (blocks > INT32_MAX) return fail(RBX_EINVAL, "rbx_cube_fwd: too many stars for one launch");
-    hipLaunchKernelGGL(star_norm_kernel, dim3(blocks), dim3(256), 0, s, g, *target, n, L.state,
-                       tps * kTile, L.nn_part);
-    hipLaunchKernelGGL(star_sf_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L.nn_part,
-                       L.state);
-  }
-  const int32_t ns = stars->n_spaxel;
-  rc = build_chunks(L.offsets, ns, L.chunk_off, L.scan_blocks, L.desc, s);
-  if (rc) return rc;
-  const int n_tiles = (nt + kTile - 1) / kTile;
-  const int tpx = (n_tiles + 7) / 8;
-  const int64_t max_chunks = (n + kChunk - 1) / kChunk + ns;
-  if (n > 0) {
-    const int64_t blocks = int64_t(8) * tpx * max_chunks;
-    if (blocks > INT32_MAX) return fail(RBX_EINVAL, "rbx_cube_fwd: problem too large for one launch");
-    hipLaunchKernelGGL(cube_accum_kernel, dim3(blocks), dim3(64), 0, s, g, *target, L.state,
-                       L.desc, L.chunk_off + ns, n_tiles, tpx, L.partial);
-  }
-  combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s);
-  return check_launch("rbx_cube_fwd");
+  rbx::Tune t;
+  rbx::tune_of(nullptr, &t);
+  return rbx::cube_fwd_impl(ssp, target, stars, t, cube, workspace, workspace_bytes,
+                            rbx::as_stream(stream), "rbx_cube_fwd");
 }
 
 int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
                  const float* dcube, float* dmass, float* dage, float* dmet, float* dvlos,
                  void* workspace, size_t workspace_bytes, void* stream) {
-  using namespace rbx;
-  int rc = validate(ssp, target, stars);
-  if (rc) return rc;
-  if (!dcube) return fail(RBX_EINVAL, "rbx_cube_bwd: null dcube");
-  hipStream_t s = as_stream(stream);
-  const int64_t n = stars->n;
-  Carver c(workspace, workspace_bytes);
-  CubeLayout L;
-  cube_layout(c, n, stars->n_keys, target->n, &L);
-  if (!c.ok) return fail(RBX_EWORKSPACE, "rbx_cube_bwd: workspace too small");
-  if (n == 0) return RBX_OK;
-  const SspView g = view_of(*ssp);
-  const int32_t nt = target->n;
-  const int n_tiles = (nt + kTile - 1) / kTile;
-  const int tps = (n_tiles + kSlabs - 1) / kSlabs;
-  const int64_t blocks = int64_t(kSlabs) * ((n + 3) / 4);
-  if (blocks > INT32_MAX) return fail(RBX_EINVAL, "rbx_cube_bwd: too many stars for one launch");
-  hipLaunchKernelGGL(star_bwd_kernel, dim3(blocks), dim3(256), 0, s, g, *target, n, L.state,
-                     dcube, tps * kTile, L.vpart, L.apart);
-  hipLaunchKernelGGL(star_grad_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *stars,
-                     L.state, L.vpart, L.apart, dmass, dage, dmet, dvlos);
-  return check_launch("rbx_cube_bwd");
+  rbx::Tune t;
+  rbx::tune_of(nullptr, &t);
+  return rbx::cube_bwd_impl(ssp, target, stars, t, dcube, dmass, dage, dmet, dvlos, workspace,
+                            workspace_bytes, rbx::as_stream(stream), "rbx_cube_bwd");
 }
 
 }  // extern "C"

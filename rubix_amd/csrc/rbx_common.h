@@ -22,6 +22,12 @@ inline int fail(int code, const char* msg) {
   return code;
 }
 
+template <class... A>
+inline int failf(int code, const char* fmt, A... args) {
+  set_error(fmt, args...);
+  return code;
+}
+
 constexpr int kWave = 64;
 
 // Round a byte count up to 256 so that workspace carving keeps alignment.

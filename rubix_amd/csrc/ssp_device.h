@@ -13,10 +13,11 @@ constexpr float kLn10 = 2.302585092994046f;
 constexpr int kChunk = 256;  // stars per reduction chunk of the segmented sums
 
 struct ChunkDesc;
-// Per-spaxel chunk descriptors from the sort offsets (cube.hip); chunk_off
-// gets n_spaxel + 1 entries, the last one the number of chunks.
-int build_chunks(const int32_t* offsets, int32_t ns, int32_t* chunk_off, int32_t* scan_blocks,
-                 ChunkDesc* desc, hipStream_t s);
+// Per-spaxel chunk descriptors (at most `chunk` stars each) from the sort
+// offsets (cube.hip); chunk_off gets n_spaxel + 1 entries, the last one the
+// number of chunks.
+int build_chunks(const int32_t* offsets, int32_t ns, int chunk, int32_t* chunk_off,
+                 int32_t* scan_blocks, ChunkDesc* desc, hipStream_t s);
 // cube[p, j] = sum of partial[c, j] over the chunks of spaxel p, in order.
 void combine_chunks(const double* partial, const int32_t* chunk_off, int32_t ns, int32_t nt,
                     float* cube, hipStream_t s);
@@ -37,7 +38,7 @@ struct SspView {
   const float* __restrict__ ax_age;
   const float* __restrict__ wave;
   const double* __restrict__ cum;
-  const int32_t* __restrict__ loc;  // n_loc pairs {k, bits of wave[k + 1]}
+  const int32_t* __restrict__ loc;  // n_loc entries {k, wave[b], wave[b+1], wave[b+2]}
   int n_met, n_age, n_wave, n_loc;
   float loc_x0, loc_inv_h;
   int log_axes;
@@ -85,7 +86,7 @@ __device__ inline int loc_cell(const SspView& g, float y) {
 // locator table gives a start within a knot or two; the walks terminate
 // after at most n steps for any input.
 __device__ inline int locate(const SspView& g, float x, float a, float inv_a) {
-  int k = g.loc[2 * loc_cell(g, x * inv_a)];
+  int k = g.loc[4 * loc_cell(g, x * inv_a)];
   const int n = g.n_wave;
   while (k + 1 < n && knot(g, k + 1, a) <= x) ++k;
   while (k >= 0 && knot(g, k, a) > x) --k;

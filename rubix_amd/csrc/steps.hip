@@ -456,7 +456,7 @@ int rbx_segment_cube(const float* spectra, const int32_t* ids, int64_t n, int32_
   if (!cv.ok) return fail(RBX_EWORKSPACE, "rbx_segment_cube: workspace too small");
   int rc = spaxel_sort_impl(ids, n, n_keys, L.perm, nullptr, L.offsets, L.sort_ws, L.sort_ws_bytes, s);
   if (rc) return rc;
-  rc = build_chunks(L.offsets, n_spaxel, L.chunk_off, L.scan_blocks, L.desc, s);
+  rc = build_chunks(L.offsets, n_spaxel, kChunk, L.chunk_off, L.scan_blocks, L.desc, s);
   if (rc) return rc;
   const int n_tiles = (n_wave + 63) / 64;
   const int64_t max_chunks = (n + kChunk - 1) / kChunk + n_spaxel;

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import RbxSsp, RbxStars, RbxTarget, call, ptr, stream_handle
+from ._lib import RbxGalaxy, RbxSsp, RbxStars, RbxTarget, RbxTuning, call, ptr, stream_handle
 
 SPEED_OF_LIGHT = 299792.458  # km/s, rubix/config/rubix_config.yml:8
 VELOCITY_COMPONENT = {"x": 0, "y": 1, "z": 2}
@@ -126,7 +126,7 @@ class PreparedSSP:
         if span * self.locate_inv_h + 2 > nloc - 1:
             self.n_locate = nloc = int(math.floor(span * self.locate_inv_h)) + 3
         self.cum = torch.empty(nm * na * nw, dtype=torch.float64, device=dev)
-        self.locate = torch.empty(2 * nloc, dtype=torch.int32, device=dev)
+        self.locate = torch.empty(4 * nloc, dtype=torch.int32, device=dev)
         self.shape = (nm, na, nw)
         self._struct = self._make_struct()
         call("rbx_ssp_prepare", self._struct, ptr(self.cum), ptr(self.locate), stream_handle(dev))
@@ -167,6 +167,7 @@ class TargetGrid:
         if w.numel() > 1:
             d[1:] = w[1:] - w[:-1]
         self.dwave = d.contiguous()
+        self.pairs = torch.stack([w, self.dwave], dim=1).contiguous()
         self.wmin = float(w.min())
         self.wmax = float(w.max())
         self.ascending = bool(torch.all(w[1:] > w[:-1])) if w.numel() > 1 else True
@@ -178,6 +179,7 @@ class TargetGrid:
         s.wmin = self.wmin
         s.wmax = self.wmax
         s.ascending = int(self.nondecreasing)
+        s.pairs = self.pairs.data_ptr()
         self._struct = s
 
     @property
@@ -303,6 +305,133 @@ def conv_axis_dot(a, b, taps, n_outer, n_axis, n_inner, start):
     res = torch.empty((), dtype=torch.float64, device=a.device)
     call("rbx_conv_axis_dot", ptr(a), ptr(b), n_outer, n_axis, n_inner, ptr(taps), taps.numel(),
          start, ptr(res), ptr(ws), wsb, stream_handle(a.device))
+    return res
+
+
+# ---------------------------------------------------------------- batches
+
+@dataclass(frozen=True)
+class Tuning:
+    """Launch tuning of the fused kernels (rbx_tuning): stars per reduction
+    chunk, wavefronts per workgroup of the per-star kernels, persistent
+    workgroups per CU (0 = one workgroup per work item)."""
+    chunk: int = 256
+    waves: int = 4
+    persist: int = 0
+
+    @property
+    def struct(self):
+        return RbxTuning(self.chunk, self.waves, self.persist, 0)
+
+
+class BatchLayout:
+    """Galaxy table of a ragged batch: galaxy g owns stars
+    [star_begin[g], star_begin[g] + n_stars[g]) of the concatenated star
+    arrays and rows [row_begin[g], row_begin[g] + nb[g]^2) of the batch cube
+    (n_rows, n_target)."""
+
+    def __init__(self, n_stars, nbs, device):
+        n_stars = [int(v) for v in n_stars]
+        nbs = [int(v) for v in nbs]
+        if len(n_stars) != len(nbs) or not n_stars:
+            raise ValueError("need one star count and one nb per galaxy")
+        if min(n_stars) < 0 or min(nbs) < 1:
+            raise ValueError("star counts must be >= 0 and nb >= 1")
+        self.n_stars = n_stars
+        self.nb = nbs
+        self.star_begin = np.concatenate([[0], np.cumsum(n_stars)[:-1]]).astype(np.int64)
+        self.row_begin = np.concatenate([[0], np.cumsum([b * b for b in nbs])[:-1]]).astype(np.int64)
+        self.n_gal = len(nbs)
+        self.n_total = int(sum(n_stars))
+        self.n_rows = int(sum(b * b for b in nbs))
+        if self.n_rows > 2 ** 31 - 3:
+            raise ValueError("batch cube has too many rows")
+        arr = (RbxGalaxy * self.n_gal)()
+        for g in range(self.n_gal):
+            arr[g].star_begin = int(self.star_begin[g])
+            arr[g].spaxel_begin = int(self.row_begin[g])
+            arr[g].nb = nbs[g]
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+        self.table = torch.from_numpy(raw).to(device)
+
+    def stars_of(self, g):
+        b = int(self.star_begin[g])
+        return slice(b, b + self.n_stars[g])
+
+    def rows_of(self, g):
+        b = int(self.row_begin[g])
+        return slice(b, b + self.nb[g] ** 2)
+
+
+def batch_forward(ssp: PreparedSSP, target: TargetGrid, layout: BatchLayout, mass, age, met,
+                  velocity, ids, tune: Tuning = None, vel_comp: int = 2, c: float = SPEED_OF_LIGHT):
+    """Fused forward of every galaxy of a batch in one pass: ids are each
+    galaxy's own spaxel ids; returns the batch cube (n_rows, n_target) and the
+    adjoint's context."""
+    _need_gpu(mass, age, met, velocity, ids)
+    mass, age, met, velocity = (_f32(mass, "mass"), _f32(age, "age"), _f32(met, "metallicity"),
+                                _f32(velocity, "velocity"))
+    if ids.dtype != torch.int32:
+        raise TypeError("ids must be int32")
+    if mass.numel() != layout.n_total:
+        raise ValueError(f"batch has {layout.n_total} stars, got {mass.numel()}")
+    ids = ids.contiguous()
+    st = _stars_struct(mass, age, met, velocity, ids, layout.n_rows, layout.n_rows + 1, vel_comp, c)
+    tune = tune or Tuning()
+    L = _lib.load()
+    wsb = L.rbx_batch_workspace_size(st.n, layout.n_rows, target.n, tune.struct)
+    if wsb == 0:
+        raise ValueError(f"invalid batch shape or tuning {tune}")
+    dev = mass.device
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    cube = torch.empty(layout.n_rows, target.n, dtype=torch.float32, device=dev)
+    call("rbx_batch_fwd", ssp.struct, target.struct, st, ptr(layout.table), layout.n_gal,
+         tune.struct, ptr(cube), ptr(ws), wsb, stream_handle(dev))
+    return cube, CubeContext(st, ws, (mass, age, met, velocity, ids))
+
+
+def batch_backward(ssp: PreparedSSP, target: TargetGrid, layout: BatchLayout, ctx: CubeContext,
+                   dcube, need=(True, True, True, True), tune: Tuning = None):
+    """Adjoint of batch_forward: (dmass, dage, dmet, dv_los) for dL/dcube."""
+    _need_gpu(dcube)
+    dcube = _f32(dcube, "dcube")
+    n = ctx.stars.n
+    dev = dcube.device
+    tune = tune or Tuning()
+    outs = [torch.empty(n, dtype=torch.float32, device=dev) if k else None for k in need]
+    call("rbx_batch_bwd", ssp.struct, target.struct, ctx.stars, ptr(layout.table), layout.n_gal,
+         tune.struct, ptr(dcube), *[ptr(o) for o in outs], ptr(ctx.workspace),
+         ctx.workspace.numel(), stream_handle(dev))
+    return tuple(outs)
+
+
+def conv_batch(x, layout: BatchLayout, axis: int, taps, start: int):
+    """One separable-PSF pass over every galaxy of a batch cube (n_rows,
+    n_target): axis 1 along x, 0 along y, zero padded inside each galaxy."""
+    _need_gpu(x, taps)
+    x = _f32(x, "cube")
+    taps = _f32(taps, "taps")
+    out = torch.empty_like(x)
+    call("rbx_conv_batch", ptr(x), ptr(out), ptr(layout.table), layout.n_gal, layout.n_rows,
+         x.shape[-1], axis, ptr(taps), taps.numel(), start, stream_handle(x.device))
+    return out
+
+
+def conv_batch_dot(a, b, layout: BatchLayout, axis: int, taps, start: int):
+    """sum(a * conv_batch(b)) as a device float64 scalar; taps float64."""
+    _need_gpu(a, b, taps)
+    a = _f32(a, "a")
+    b = _f32(b, "b")
+    if taps.dtype != torch.float64:
+        raise TypeError("conv_batch_dot taps must be float64")
+    taps = taps.contiguous()
+    L = _lib.load()
+    wsb = L.rbx_dot_workspace_size(a.numel())
+    ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=a.device)
+    res = torch.empty((), dtype=torch.float64, device=a.device)
+    call("rbx_conv_batch_dot", ptr(a), ptr(b), ptr(layout.table), layout.n_gal, layout.n_rows,
+         a.shape[-1], axis, ptr(taps), taps.numel(), start, ptr(res), ptr(ws), wsb,
+         stream_handle(a.device))
     return res
 
 

@@ -178,11 +178,29 @@ def config2(n_stars=1_000_000, seed=0):
 
 
 def config5(n_stars=10_000_000, seed=0):
-    """Large galaxy: 1e7 stars into 150x150 x 3721 with the small SSP grid."""
+    """Large galaxy: 1e7 stars into 150x150 spaxels at 0.2" x 3721 bins with
+    the small SSP grid (6 Z x 20 ages x 1000 l on 3000-9000 A, z = 0.05) and
+    the small workload's PSF (sigma 0.6" on 11x11) and LSF (sigma 1.2 A)."""
     stars = make_stars(n_stars, seed=seed)
     grid = make_ssp_grid(6, 20, 1000, (3000.0, 9000.0), seed=1)
-    geom = make_geometry(150, 0.2, 0.05, muse_wave(), 0.8, 15, 1.0, 1.25)
+    geom = make_geometry(150, 0.2, 0.05, muse_wave(), 0.6, 11, 1.2, 1.25)
     return Workload("cfg5", stars, grid, geom)
+
+
+def config4(n_gal=64, seed=3, scale=1.0):
+    """Batched ragged workload: n_gal galaxies with star counts log-uniform on
+    5e4..2e6 (times scale) and spaxel grids from {25, 50, 100} at 0.2", the
+    shared 3721-bin axis and the small SSP grid (z = 0.05), PSF sigma 0.6" on
+    11x11, LSF sigma 1.2 A.  Galaxy g's stars use seed 100 + g."""
+    ns, nbs = config4_sizes(n_gal, seed)
+    grid = make_ssp_grid(6, 20, 1000, (3000.0, 9000.0), seed=1)
+    out = []
+    for g, (n, nb) in enumerate(zip(ns, nbs)):
+        n = max(1, int(round(n * scale)))
+        stars = make_stars(n, seed=100 + g)
+        geom = make_geometry(int(nb), 0.2, 0.05, muse_wave(), 0.6, 11, 1.2, 1.25)
+        out.append(Workload(f"cfg4-{g}", stars, grid, geom))
+    return out
 
 
 def config4_sizes(n_gal=64, seed=3):

@@ -28,10 +28,13 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_static_asserts():
-    # cube.hip static_asserts sizeof(rbx_ssp) == 80, rbx_target == 32, rbx_stars == 80
+    # cube.hip static_asserts sizeof(rbx_ssp) == 80, rbx_target == 40, rbx_stars == 80
     assert ctypes.sizeof(_lib.RbxSsp) == 80
-    assert ctypes.sizeof(_lib.RbxTarget) == 32
+    assert ctypes.sizeof(_lib.RbxTarget) == 40
     assert ctypes.sizeof(_lib.RbxStars) == 80
+    # batch.hip / conv.hip index rbx_galaxy arrays on the device: 16 bytes each
+    assert ctypes.sizeof(_lib.RbxGalaxy) == 16
+    assert ctypes.sizeof(_lib.RbxTuning) == 16
 
 
 def test_version_and_errors():
@@ -52,6 +55,23 @@ def test_version_and_errors():
         _lib.call("rbx_cube_fwd", s, t, st, None, None, 0, None)
     with pytest.raises(_lib.RbxError, match="in-place"):
         _lib.call("rbx_conv_axis", 8, 8, 1, 4, 1, 16, 3, 1, None)
+
+
+def test_batch_arguments_are_validated_before_any_gpu_call():
+    L = _lib.load()
+    bad = _lib.RbxTuning(3, 0, 0, 0)  # chunk below 16
+    assert L.rbx_batch_workspace_size(1000, 100, 50, bad) == 0
+    assert L.rbx_batch_workspace_size(1000, 100, 50, None) > L.rbx_cube_workspace_size(1000, 101, 50)
+    with pytest.raises(_lib.RbxError, match="waves must be"):
+        st = _lib.RbxStars()
+        st.n_spaxel = 4
+        _lib.call("rbx_batch_fwd", _lib.RbxSsp(), _lib.RbxTarget(), st, 64, 1,
+                  _lib.RbxTuning(0, 3, 0, 0), None, None, 0, None)
+    with pytest.raises(_lib.RbxError, match="galaxy table"):
+        _lib.call("rbx_batch_bwd", _lib.RbxSsp(), _lib.RbxTarget(), _lib.RbxStars(), None, 0, None,
+                  None, None, None, None, None, None, 0, None)
+    with pytest.raises(_lib.RbxError, match="rbx_conv_batch"):
+        _lib.call("rbx_conv_batch", 8, 16, 24, 1, 4, 10, 2, 32, 3, 1, None)
 
 
 def test_workspace_queries():
