@@ -303,6 +303,46 @@ int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer,
                       int32_t n_taps, int32_t start, double* result,
                       void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------- alignment
+ * The steps before spaxel assignment in the reference chain
+ * (rubix/config/pipeline_config.yml: rotate_galaxy -> filter_particles).
+ *
+ * rbx_filter_particles replaces mask_particles_outside_aperture +
+ * filter_particles (rubix/telescope/utils.py:155-178,
+ * rubix/core/telescope.py:129-201): mask = coords x and y within [lo, hi]
+ * (the min / max spatial bin edge, inclusive); each non-NULL *_out gets the
+ * field where the mask holds and 0 elsewhere (may alias its input); mask_out
+ * (uint8, may be NULL) the mask.  rbx_apply_mask: out = mask ? in : 0 over
+ * (n, width) rows (the adjoint of the filter).
+ *
+ * rbx_inertia_tensor replaces moment_of_inertia_tensor
+ * (rubix/galaxy/alignment.py:66-120): out9 (device, row-major 3x3 double)
+ * = sum over particles with sqrt(x^2+y^2+z^2) <= halfmass_radius (float32
+ * test) of m (|r|^2 delta_ij - r_i r_j), fixed-order double reduction;
+ * workspace rbx_inertia_workspace_size(n) bytes.
+ *
+ * rbx_rotate_rows replaces apply_init_rotation + apply_rotation
+ * (alignment.py:123-137,208-225): rows (n, stride >= 3; columns past 2 are
+ * copied) out = (in first) second with row-major 3x3 float matrices, each
+ * product in float32 without fused multiply-adds. */
+typedef struct rbx_rotation {
+  float first[9];
+  float second[9];
+} rbx_rotation;
+
+int rbx_filter_particles(const float* coords, int64_t n, int64_t stride,
+                         float lo, float hi, const float* mass, const float* age,
+                         const float* metallicity, float* mass_out, float* age_out,
+                         float* metallicity_out, uint8_t* mask_out, void* stream);
+int rbx_apply_mask(const uint8_t* mask, int64_t n, int64_t width,
+                   const float* in, float* out, void* stream);
+size_t rbx_inertia_workspace_size(int64_t n);
+int rbx_inertia_tensor(const float* coords, const float* mass, int64_t n,
+                       int64_t stride, float halfmass_radius, double* out9,
+                       void* workspace, size_t workspace_bytes, void* stream);
+int rbx_rotate_rows(const float* in, int64_t n, int64_t stride,
+                    const rbx_rotation* rot, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

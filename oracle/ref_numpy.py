@@ -29,6 +29,16 @@ Pinning (what checks each function, in tests/test_oracle.py):
   reference delta -> normalised Gaussian known answer
   (tests/test_telescope_lsf.py:6-60).
 * gaussian_kernel_2d       - sums to 1, centre max, equals outer(g, g).
+* mask_particles_outside_aperture, filter_particles - reference boundary
+  cases (tests/test_telescope_utils.py:38-79) + brute force.
+* moment_of_inertia_tensor - reference known answer diag(2, 2, 2)
+  (tests/test_galaxy_alignment.py:76-90) + brute-force double loop.
+* rotation_matrix_from_inertia_tensor - diagonalises I (R^T I R diagonal,
+  eigenvalues ascending), identity for diag(1, 2, 3)
+  (tests/test_galaxy_alignment.py:93-108).
+* euler_rotation_matrix    - reference known answer for (90, 0, 0)
+  (tests/test_galaxy_alignment.py:132-146), scipy Rotation.from_euler
+  composition, orthonormal with det 1.
 """
 from __future__ import annotations
 
@@ -76,6 +86,63 @@ def mask_particles_outside_aperture(coords, edges):
     m = (coords[:, 0] >= lo) & (coords[:, 0] <= hi)
     m &= (coords[:, 1] >= lo) & (coords[:, 1] <= hi)
     return m
+
+
+def filter_particles(coords, edges, fields):
+    """rubix/core/telescope.py:129-201: every per-particle field except coords
+    and velocity becomes jnp.where(mask, value, 0); returns (fields, mask)."""
+    mask = mask_particles_outside_aperture(coords, edges)
+    out = {k: np.where(mask, np.asarray(v), np.zeros((), dtype=np.asarray(v).dtype))
+           for k, v in fields.items()}
+    return out, mask
+
+
+def moment_of_inertia_tensor(positions, masses, halfmass_radius):
+    """rubix/galaxy/alignment.py:66-120: I_ii = sum m (|r|^2 - r_i^2),
+    I_ij = -sum m r_i r_j over particles with |r| <= halfmass_radius (the
+    radius test in float32 as the reference evaluates it; sums in float64)."""
+    p32 = np.asarray(positions, dtype=np.float32)
+    d = np.sqrt((p32[:, 0] * p32[:, 0] + p32[:, 1] * p32[:, 1]) + p32[:, 2] * p32[:, 2])
+    keep = d <= np.float32(halfmass_radius)
+    p = p32[keep].astype(np.float64)
+    m = np.asarray(masses, dtype=np.float64)[keep]
+    q = np.sum(p * p, axis=1)
+    I = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            if i == j:
+                I[i, j] = np.sum(m * q - m * p[:, i] ** 2)
+            else:
+                I[i, j] = -np.sum(m * p[:, i] * p[:, j])
+    return I
+
+
+def rotation_matrix_from_inertia_tensor(I):
+    """rubix/galaxy/alignment.py:123-137: eigenvectors of I (eigh) as columns,
+    ordered by ascending eigenvalue."""
+    vals, vecs = np.linalg.eigh(np.asarray(I, dtype=np.float64))
+    return vecs[:, np.argsort(vals)]
+
+
+def euler_rotation_matrix(alpha, beta, gamma):
+    """rubix/galaxy/alignment.py:140-205: R = Rz(gamma) Ry(beta) Rx(alpha),
+    angles in degrees, elementary rotations about the fixed axes."""
+    a, b, c = np.deg2rad([alpha, beta, gamma])
+    Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    Ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+    Rz = np.array([[np.cos(c), -np.sin(c), 0], [np.sin(c), np.cos(c), 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def rotate_galaxy(positions, velocities, masses, halfmass_radius, alpha, beta, gamma):
+    """rubix/galaxy/alignment.py:240-265: p' = (p R) E with R the principal
+    axes of the inertia tensor and E the Euler matrix (row vectors)."""
+    R = rotation_matrix_from_inertia_tensor(
+        moment_of_inertia_tensor(positions, masses, halfmass_radius))
+    E = euler_rotation_matrix(alpha, beta, gamma)
+    p = np.asarray(positions, dtype=np.float64) @ R @ E
+    v = np.asarray(velocities, dtype=np.float64) @ R @ E
+    return p, v
 
 
 def sort_by_spaxel(ids, n_seg):

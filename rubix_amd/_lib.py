@@ -49,6 +49,10 @@ class RbxTuning(ctypes.Structure):
                 ("reserved", c_int32)]
 
 
+class RbxRotation(ctypes.Structure):
+    _fields_ = [("first", c_float * 9), ("second", c_float * 9)]
+
+
 # name -> (restype, argtypes); mirrors include/rubix_amd.h
 SIGNATURES = {
     "rbx_version": (c_int, []),
@@ -75,6 +79,15 @@ SIGNATURES = {
     "rbx_conv_batch_dot": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
                                    c_int32, c_void_p, c_int32, c_int32, c_void_p, c_void_p,
                                    c_size_t, c_void_p]),
+    "rbx_filter_particles": (c_int, [c_void_p, c_int64, c_int64, c_float, c_float, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    "rbx_apply_mask": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    "rbx_inertia_workspace_size": (c_size_t, [c_int64]),
+    "rbx_inertia_tensor": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_float, c_void_p,
+                                   c_void_p, c_size_t, c_void_p]),
+    "rbx_rotate_rows": (c_int, [c_void_p, c_int64, c_int64, POINTER(RbxRotation), c_void_p,
+                                c_void_p]),
     "rbx_ssp_lookup": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "rbx_ssp_lookup_bwd": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),

@@ -204,6 +204,41 @@ class GaussianLSF(torch.autograd.Function):
         return gin, dsig, None, None
 
 
+# ---------------------------------------------------------------- alignment
+
+class FilterFields(torch.autograd.Function):
+    """filter_particles on up to three per-particle fields; the adjoint masks
+    the incoming gradients the same way."""
+
+    @staticmethod
+    def forward(ctx, coords, lo, hi, *fields):
+        *outs, mask = ops.filter_particles(coords.detach(), lo, hi, *[f.detach() for f in fields])
+        ctx.save_for_backward(mask)
+        ctx.mark_non_differentiable(mask)
+        return (*outs, mask)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        (mask,) = ctx.saved_tensors
+        outs = [None if g is None else ops.apply_mask(mask, g.contiguous()) for g in grads[:-1]]
+        return (None, None, None, *outs)
+
+
+class RotateRows(torch.autograd.Function):
+    """Row vectors x -> (x first) second; adjoint g -> (g second^T) first^T.
+    The matrices are constants (the alignment frame is not differentiated)."""
+
+    @staticmethod
+    def forward(ctx, x, first, second):
+        ctx.mats = (np.asarray(first, np.float64), np.asarray(second, np.float64))
+        return ops.rotate_rows(x.detach(), first, second)
+
+    @staticmethod
+    def backward(ctx, g):
+        first, second = ctx.mats
+        return ops.rotate_rows(g.contiguous(), second.T, first.T), None, None
+
+
 # ---------------------------------------------------------------- batches
 
 class BatchedDatacube(torch.autograd.Function):

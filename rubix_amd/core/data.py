@@ -129,12 +129,14 @@ def get_reshape_data(config: dict) -> Callable:
 
 
 def stars_from_arrays(coords, velocity, mass, age, metallicity, device="cuda",
-                      requires_grad=False) -> RubixData:
-    """Convenience: RubixData with the star fields as float32 device tensors."""
+                      requires_grad=False, halfmassrad_stars=None) -> RubixData:
+    """Convenience: RubixData with the star fields as float32 device tensors
+    (and the galaxy's stellar half-mass radius, which rotate_galaxy reads)."""
     def t(x):
         v = torch.as_tensor(np.asarray(x, dtype=np.float32), device=device)
         return v.requires_grad_(requires_grad)
 
-    return RubixData(stars=StarsData(coords=torch.as_tensor(np.asarray(coords, np.float32), device=device),
+    return RubixData(galaxy=Galaxy(halfmassrad_stars=halfmassrad_stars),
+                     stars=StarsData(coords=torch.as_tensor(np.asarray(coords, np.float32), device=device),
                                      velocity=t(velocity), mass=t(mass), age=t(age),
                                      metallicity=t(metallicity)))

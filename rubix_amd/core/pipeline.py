@@ -1,8 +1,11 @@
 """Step-chain runner: builds the registered step closures for a named chain
-of the package's pipeline_config.yml ("calc_ifu": the per-step chain,
-"calc_ifu_fused": with the fused datacube step) and applies them to a
-RubixData, the part of RubixPipeline.run (rubix/core/pipeline.py:93-201)
-that covers this path."""
+of the package's pipeline_config.yml and applies them to a RubixData, the
+part of RubixPipeline.run (rubix/core/pipeline.py:93-201) that covers this
+path.  Chains: "calc_ifu" (rotate_galaxy -> filter_particles -> the
+per-step chain -> convolve_lsf, the reference's chain up to its noise step),
+"calc_ifu_fused" (the same with the fused datacube step), and
+"calc_ifu_aligned" / "calc_ifu_fused_aligned" starting at spaxel_assignment
+for particles that are already aligned and filtered."""
 from __future__ import annotations
 
 from ..config import pipeline_config
@@ -12,9 +15,12 @@ from .ifu import (get_calculate_datacube, get_calculate_datacube_fused, get_calc
                   get_doppler_shift_and_resampling, get_scale_spectrum_by_mass)
 from .lsf import get_convolve_lsf
 from .psf import get_convolve_psf
-from .telescope import get_spaxel_assignment
+from .rotation import get_galaxy_rotation
+from .telescope import get_filter_particles, get_spaxel_assignment
 
 FACTORIES = {
+    "rotate_galaxy": get_galaxy_rotation,
+    "filter_particles": get_filter_particles,
     "spaxel_assignment": get_spaxel_assignment,
     "reshape_data": get_reshape_data,
     "calculate_spectra": get_calculate_spectra,

@@ -13,7 +13,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import RbxGalaxy, RbxSsp, RbxStars, RbxTarget, RbxTuning, call, ptr, stream_handle
+from ._lib import (RbxGalaxy, RbxRotation, RbxSsp, RbxStars, RbxTarget, RbxTuning, call, ptr,
+                   stream_handle)
 
 SPEED_OF_LIGHT = 299792.458  # km/s, rubix/config/rubix_config.yml:8
 VELOCITY_COMPONENT = {"x": 0, "y": 1, "z": 2}
@@ -69,6 +70,74 @@ def spaxel_sort(ids: torch.Tensor, n_keys: int):
     call("rbx_spaxel_sort", ptr(ids), n, n_keys, ptr(perm), ptr(srt), ptr(offsets), ptr(ws), wsb,
          stream_handle(dev))
     return perm, srt, offsets
+
+
+# ---------------------------------------------------------------- alignment
+
+def filter_particles(coords, lo: float, hi: float, *fields):
+    """filter_particles (rubix/core/telescope.py:129-201): mask = coords x, y
+    within [lo, hi]; each field (N,) float32 becomes field where mask else 0.
+    Returns (masked fields..., mask (N,) bool)."""
+    _need_gpu(coords, *fields)
+    coords = _f32(coords, "coords")
+    if len(fields) > 3:
+        raise ValueError("at most three fields per call")
+    n = coords.shape[0]
+    ins = [_f32(f, "field") for f in fields]
+    for f in ins:
+        if f.numel() != n:
+            raise ValueError("fields must have one value per particle")
+    outs = [torch.empty_like(f) for f in ins]
+    mask = torch.empty(n, dtype=torch.uint8, device=coords.device)
+    pad_in = ins + [None] * (3 - len(ins))
+    pad_out = outs + [None] * (3 - len(outs))
+    call("rbx_filter_particles", ptr(coords), n, coords.shape[1], lo, hi, *[ptr(t) for t in pad_in],
+         *[ptr(t) for t in pad_out], ptr(mask), stream_handle(coords.device))
+    return (*outs, mask.bool())
+
+
+def apply_mask(mask, x):
+    """x where mask else 0, row-wise for (N, ...) x (the filter's adjoint)."""
+    _need_gpu(mask, x)
+    x = _f32(x, "x")
+    m = mask.to(torch.uint8).contiguous()
+    n = m.numel()
+    out = torch.empty_like(x)
+    call("rbx_apply_mask", ptr(m), n, x.numel() // max(n, 1), ptr(x), ptr(out),
+         stream_handle(x.device))
+    return out
+
+
+def inertia_tensor(coords, mass, halfmass_radius: float):
+    """moment_of_inertia_tensor (rubix/galaxy/alignment.py:66-120) as a
+    (3, 3) float64 device tensor (fixed-order double reduction)."""
+    _need_gpu(coords, mass)
+    coords = _f32(coords, "coords")
+    mass = _f32(mass, "mass")
+    n = coords.shape[0]
+    L = _lib.load()
+    wsb = L.rbx_inertia_workspace_size(n)
+    ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=coords.device)
+    out = torch.empty(3, 3, dtype=torch.float64, device=coords.device)
+    call("rbx_inertia_tensor", ptr(coords), ptr(mass), n, coords.shape[1], float(halfmass_radius),
+         ptr(out), ptr(ws), wsb, stream_handle(coords.device))
+    return out
+
+
+def rotate_rows(x, first, second):
+    """(x first) second for row vectors x (N, >=3) with 3x3 matrices
+    (apply_init_rotation then apply_rotation, rubix/galaxy/alignment.py)."""
+    _need_gpu(x)
+    x = _f32(x, "x")
+    rot = RbxRotation()
+    a = np.asarray(first, dtype=np.float32).reshape(9)
+    b = np.asarray(second, dtype=np.float32).reshape(9)
+    for k in range(9):
+        rot.first[k] = float(a[k])
+        rot.second[k] = float(b[k])
+    out = torch.empty_like(x)
+    call("rbx_rotate_rows", ptr(x), x.shape[0], x.shape[1], rot, ptr(out), stream_handle(x.device))
+    return out
 
 
 # ---------------------------------------------------------------- SSP tables

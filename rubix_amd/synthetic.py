@@ -232,7 +232,8 @@ def step_config(w: Workload, device="cuda"):
            "pixel_type": "square"}
     return {
         "device": device,
-        "galaxy": {"dist_z": g["z"]},
+        "galaxy": {"dist_z": g["z"], "rotation": {"type": "face-on"}},
+        "data": {"args": {"particle_type": ["stars"]}},
         "telescopes": {f"SYNTHETIC_{w.name}": tel},
         "telescope": {"name": f"SYNTHETIC_{w.name}", "spatial_bin_edges": g["edges"],
                       "wave_seq": g["wave_target"],

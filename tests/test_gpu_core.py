@@ -1,7 +1,9 @@
-"""GPU: the drop-in step API end to end.  The per-step chain (calc_ifu) and
-the fused chain (calc_ifu_fused) run through RubixPipeline on RubixData and
-are checked against the float64 oracle; every step writes the reference's
-fields with the reference's shapes."""
+"""GPU: the drop-in step API end to end.  The per-step chain and the fused
+chain run through RubixPipeline on RubixData and are checked against the
+float64 oracle: from spaxel assignment (calc_ifu*_aligned) and from raw
+particles through rotate_galaxy and filter_particles (calc_ifu,
+calc_ifu_fused).  Every step writes the reference's fields with the
+reference's shapes."""
 import numpy as np
 import pytest
 import torch
@@ -25,7 +27,7 @@ def _data(w, requires_grad=False):
                              requires_grad=requires_grad)
 
 
-@pytest.mark.parametrize("chain", ["calc_ifu", "calc_ifu_fused"])
+@pytest.mark.parametrize("chain", ["calc_ifu_aligned", "calc_ifu_fused_aligned"])
 def test_pipeline_matches_oracle(chain):
     w = cfg1(4000)
     cfg = S.step_config(w)
@@ -34,8 +36,40 @@ def test_pipeline_matches_oracle(chain):
     cube = out.stars.datacube
     assert tuple(cube.shape) == (25, 25, 1000)
     assert rel_err(cube.detach().cpu().numpy(), ref) <= TOL
-    if chain == "calc_ifu_fused":
+    if chain == "calc_ifu_fused_aligned":
         assert out.stars.spectra is None
+
+
+@pytest.mark.parametrize("chain,rotation", [("calc_ifu", {"type": "face-on"}),
+                                            ("calc_ifu_fused", {"type": "edge-on"}),
+                                            ("calc_ifu_fused", {"alpha": 45.0, "beta": 30.0,
+                                                                "gamma": 15.0})])
+def test_full_chain_from_raw_particles(chain, rotation):
+    """rotate_galaxy -> filter_particles -> ... -> convolve_lsf against the
+    oracle's alignment, aperture filter and plain forward (float64)."""
+    w = S.config1(n_stars=4000, seed=7)
+    cfg = S.step_config(w)
+    cfg["galaxy"]["rotation"] = rotation
+    r_half = 4.0
+    st = w.stars
+    rd = stars_from_arrays(st["coords"], st["velocity"], st["mass"], st["age"], st["metallicity"],
+                           halfmassrad_stars=r_half)
+    out = RubixPipeline(cfg, chain).run(rd)
+    if "type" in rotation:
+        ang = (0.0, 0.0, 0.0) if rotation["type"] == "face-on" else (90.0, 0.0, 0.0)
+    else:
+        ang = (rotation["alpha"], rotation["beta"], rotation["gamma"])
+    p, v = R.rotate_galaxy(st["coords"], st["velocity"], st["mass"], r_half, *ang)
+    fields, mask = R.filter_particles(p.astype(np.float32), w.geom["edges"],
+                                      {k: st[k] for k in ("mass", "age", "metallicity")})
+    ref_stars = dict(coords=p.astype(np.float32), velocity=v.astype(np.float32), **fields)
+    ref = R.plain_forward(ref_stars, w.grid, w.geom, dtype=np.float64)
+    got_p = out.stars.coords.detach().cpu().numpy().reshape(-1, 3)
+    assert np.abs(got_p - p).max() <= 1e-5 * np.abs(p).max()
+    np.testing.assert_array_equal(out.stars.mask.cpu().numpy().reshape(-1), mask)
+    e = rel_err(out.stars.datacube.detach().cpu().numpy(), ref)
+    print(f"{chain} {rotation}: rel64={e:.3e}, {int(mask.sum())} of {len(mask)} inside")
+    assert e <= TOL
 
 
 def test_steps_fields_and_shapes():
@@ -64,7 +98,7 @@ def test_pipeline_gradient_fused_equals_plain():
     w = cfg1(500)
     cfg = S.step_config(w)
     grads = []
-    for chain in ("calc_ifu", "calc_ifu_fused"):
+    for chain in ("calc_ifu_aligned", "calc_ifu_fused_aligned"):
         rd = _data(w, requires_grad=True)
         leaves = (rd.stars.mass, rd.stars.age, rd.stars.metallicity, rd.stars.velocity)
         p = RubixPipeline(cfg, chain)
