@@ -178,7 +178,7 @@ def run_single(args):
 
     with torch.no_grad():
         # fixed target: a second galaxy (seed 2) of the same size, S/N = 30 weights
-        st2 = S.make_stars(args.n_stars, seed=2)
+        st2 = S.make_stars(w.n_stars, seed=2)
         t2 = {k: torch.as_tensor(v, device=dev) for k, v in st2.items()}
         tgt = fused(t2["mass"], t2["age"], t2["metallicity"], t2["velocity"]).clone()
         inv_var = 1.0 / torch.clamp((tgt.abs() / 30.0) ** 2, min=(1e-6 * float(tgt.abs().max())) ** 2)
