@@ -24,6 +24,11 @@ if ROOT not in sys.path:
 
 METRIC = "star·λ-bins/s fwd+adjoint, 1M stars→50²×3721 cube; % HBM roofline vs plain"
 HBM_PEAK_GBS = 8000.0
+# Physical HBM bytes of one rbx_cube_bwd launch (star_bwd_kernel +
+# star_grad_kernel) at 1M stars -> 50x50x3721 with the 12x100x5000 grid:
+# rocprofv3 FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), the factors from a 1 GiB
+# device copy (bench.py --calibrate-copy); profiles/hbm_traffic_pmc_1M_50x50x3721.md.
+TRAFFIC_BWD_1M_BYTES = 3.217e9
 
 
 def log(*a):
@@ -218,8 +223,10 @@ def run_single(args):
     bwd_ms = per_call.get("rbx_cube_bwd", float("nan"))
     bwd_bytes = n * (24 * n_src + 8 * nt)
     achieved = bwd_bytes / (bwd_ms * 1e-3) / 1e9
+    measured = (n, nb, nt, n_src) == (1_000_000, 50, 3721, 5000)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": TRAFFIC_BWD_1M_BYTES if measured else None,
                 "kernel": "rbx_cube_bwd (star_bwd_kernel + star_grad_kernel)",
                 "kernel_ms": round(bwd_ms, 4), "algorithmic_bytes_per_launch": bwd_bytes,
                 "step_effective_frac": round(step_frac, 4)}

@@ -192,12 +192,18 @@ __device__ inline Knots knots_of(const SspView& g, float x, const Shift& h) {
   lp.lo = hi ? e2 : e1;
   lp.hi = hi ? e3 : e2;
   float wl = mul_nocontract(lp.lo, h.a), wr = mul_nocontract(lp.hi, h.a);
-  // exact segment: i in [1, n-1] moves monotonically, so the loop ends
-  while ((x >= wr && i < n - 1) || (x < wl && i > 1)) {
-    i += x >= wr ? 1 : -1;
-    lp = ld_pair(g.wave, uint32_t(i - 1));
-    wl = mul_nocontract(lp.lo, h.a);
-    wr = mul_nocontract(lp.hi, h.a);
+  // exact segment: i in [1, n-1] moves monotonically, so the loop ends.
+  // Corrections are rare; a wave vote keeps the common case to one
+  // compare and a uniform branch.
+  bool step = (x >= wr && i < n - 1) || (x < wl && i > 1);
+  if (__ballot(step)) {
+    while (step) {
+      i += x >= wr ? 1 : -1;
+      lp = ld_pair(g.wave, uint32_t(i - 1));
+      wl = mul_nocontract(lp.lo, h.a);
+      wr = mul_nocontract(lp.hi, h.a);
+      step = (x >= wr && i < n - 1) || (x < wl && i > 1);
+    }
   }
   const uint32_t i0 = uint32_t(i - 1);
   const float l0 = lp.lo, l1 = lp.hi;
@@ -351,10 +357,10 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
   const int n_chunks = *n_chunks_ptr;
   for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
     const int xcd = int(q & 7);
-    const int64_t w = q >> 3;
-    const int tile = xcd * tiles_per_xcd + int(w % tiles_per_xcd);
-    const int64_t chunk = w / tiles_per_xcd;
-    if (tile >= n_tiles || chunk >= n_chunks) continue;
+    const uint32_t w = uint32_t(q >> 3);  // n_items < 2^34 (checked on the host)
+    const uint32_t chunk = w / uint32_t(tiles_per_xcd);
+    const int tile = xcd * tiles_per_xcd + int(w - chunk * uint32_t(tiles_per_xcd));
+    if (tile >= n_tiles || int64_t(chunk) >= n_chunks) continue;
     const ChunkDesc d = desc[chunk];
     const int j = tile * kTile + int(threadIdx.x);
     const bool valid = j < tg.n;
@@ -371,7 +377,7 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
       const Knots K = knots_of<false>(g, x, shift_of(g, s.a, s.inv_a));
       acc += double(star_value(R, s, K) * s.sf);
     }
-    if (valid) partial[chunk * tg.n + j] = acc;
+    if (valid) partial[int64_t(chunk) * tg.n + j] = acc;
   }
 }
 
@@ -706,18 +712,22 @@ static int n_cus() {
   return cache[dev];
 }
 
-// Workgroups for `items` slab-mapped work items (a multiple of 8): one per
-// item, or a persistent grid of `persist` workgroups per CU; always a
-// multiple of 8.  0 if the grid does not fit one launch.
-static int64_t item_grid(int64_t items, int persist) {
+// Workgroups of `threads` for `items` slab-mapped work items (a multiple of
+// 8): one per item, or a persistent grid of `persist` workgroups per CU;
+// always a multiple of 8, and capped so that the launch has fewer than 2^32
+// threads (one dispatch packet, also when captured into a graph); the
+// kernels stride over the remaining items.
+static int64_t item_grid(int64_t items, int persist, int threads) {
   int64_t g = items;
   if (persist > 0) {
     const int64_t p = int64_t(persist) * n_cus();
     if (p < g) g = p;
   }
+  const int64_t cap = ((int64_t(1) << 32) - 1) / threads / 8 * 8;
+  if (g > cap) g = cap;
   g = (g + 7) / 8 * 8;
   if (g < 8) g = 8;
-  return g > INT32_MAX ? 0 : g;
+  return g;
 }
 
 template <int W>
@@ -804,14 +814,14 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
     hipLaunchKernelGGL(star_prep_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *target,
                        *stars, L.perm, L.skey, L.state);
     const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
-    const int64_t grid = item_grid(items, t.persist);
-    if (!grid) return failf(RBX_EINVAL, "%s: too many stars for one launch", who);
+    const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
     switch (t.waves) {
       case 1: launch_norm<1>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part); break;
       case 2: launch_norm<2>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part); break;
       case 8: launch_norm<8>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part); break;
       default: launch_norm<4>(grid, s, g, *target, n, items, L.state, tps * kTile, L.nn_part);
     }
+    if ((rc = check_launch(who))) return rc;
     hipLaunchKernelGGL(star_sf_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L.nn_part,
                        L.state);
   }
@@ -821,10 +831,12 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   const int tpx = (n_tiles + 7) / 8;
   if (n > 0) {
     const int64_t items = int64_t(8) * tpx * L.max_chunks;
-    const int64_t grid = item_grid(items, t.persist);
-    if (!grid) return failf(RBX_EINVAL, "%s: problem too large for one launch", who);
+    if (items >= (int64_t(1) << 34))
+      return failf(RBX_EINVAL, "%s: problem too large for one launch", who);
+    const int64_t grid = item_grid(items, t.persist, 64);
     hipLaunchKernelGGL(cube_accum_kernel, dim3(grid), dim3(64), 0, s, g, *target, L.state,
                        L.desc, L.chunk_off + ns, n_tiles, tpx, items, L.partial);
+    if ((rc = check_launch(who))) return rc;
   }
   combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s);
   return check_launch(who);
@@ -848,14 +860,14 @@ int cube_bwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   const int n_tiles = (nt + kTile - 1) / kTile;
   const int tps = (n_tiles + kSlabs - 1) / kSlabs;
   const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
-  const int64_t grid = item_grid(items, t.persist);
-  if (!grid) return failf(RBX_EINVAL, "%s: too many stars for one launch", who);
+  const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
   switch (t.waves) {
     case 1: launch_bwd<1>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
     case 2: launch_bwd<2>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
     case 8: launch_bwd<8>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
     default: launch_bwd<4>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart);
   }
+  if ((rc = check_launch(who))) return rc;
   hipLaunchKernelGGL(star_grad_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *stars,
                      L.state, L.vpart, L.apart, dmass, dage, dmet, dvlos);
   return check_launch(who);
