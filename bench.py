@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="timed steps of the plain per-step GPU path (0 = skip)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched workload: default launch tuning, no autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (twice): the reference "
                          "workload for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -430,7 +432,10 @@ def run_batched(args):
 
     key = autotune.problem_key(L.n_total, L.n_rows, nt, n_src, dev)
     t0 = time.perf_counter()
-    best, seen = autotune.autotune(eager_step, key, reps=1, log=log)
+    if args.no_tune:
+        best, seen = ops.Tuning(), {}
+    else:
+        best, seen = autotune.autotune(eager_step, key, reps=1, log=log)
     log(f"autotune {time.perf_counter() - t0:.1f}s -> {best}")
     state["tune"] = best
     eager_ms = autotune.time_ms(eager_step, best, reps=3)

@@ -70,7 +70,9 @@ int rbx_spaxel_sort(const int32_t* ids, int64_t n, int32_t n_keys,
  * hold log10 values and queries are log10'd before interpolation.
  * rbx_ssp_prepare fills cum (double, n_met*n_age*n_wave: per-template
  * prefix sums of flux[k]*(wave_obs[k]-wave_obs[k-1]), 0 at k = 0) and the
- * knot locator, n_locate entries of 4 int32.  With h = 1/locate_inv_h, entry
+ * knot locator: n_locate entries of 4 int32 followed by the n_locate k
+ * values again as a compact int32 array (5 n_locate int32 in all).  With
+ * h = 1/locate_inv_h, entry
  * m >= 1 describes the cell starting at locate_x0 + (m-1) h and entry 0 every
  * query below locate_x0: {k, bits of wave_obs[b], wave_obs[b+1],
  * wave_obs[b+2] as float} with k = (number of wave_obs <= cell start) - 1

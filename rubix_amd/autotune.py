@@ -3,8 +3,8 @@
 The tunables are the stars per reduction chunk of the cube accumulation
 (more chunks = more parallel work items but more partial rows to combine),
 the wavefronts per workgroup of the per-star kernels, and whether those
-kernels run as one workgroup per work item or as a persistent grid of a few
-workgroups per compute unit.  Which is best depends on the star count, the
+kernels run as one workgroup per work item or as a persistent grid of
+workgroups per compute unit (8 or more: fewer leave the CUs underoccupied).  Which is best depends on the star count, the
 number of spaxels (stars per spaxel sets the chunk lengths) and the
 wavelength axes, so the tuner times the caller's own step on the device and
 caches the winner per (device, problem shape) in a JSON file.
@@ -27,7 +27,7 @@ from .ops import Tuning
 
 CANDIDATES = {
     "waves": (1, 2, 4, 8),
-    "persist": (0, 2, 4, 8),
+    "persist": (0, 8, 16, 32),
     "chunk": (64, 128, 256, 512, 1024),
 }
 

@@ -7,11 +7,17 @@
 // (rubix/telescope/lsf/lsf.py:29-105) is a 25-tap full convolution along
 // lambda cropped at the extend factor.  Both are one kernel: a 1D
 // convolution along the middle axis of an (n_outer, n_axis, n_inner) view.
-// Threads map to the flattened output in row-major order, so loads and
-// stores are contiguous along lambda (the innermost, contiguous axis) for
-// the spatial passes and along the convolved axis for the LSF.  The cube is
-// 37 MB at 50x50x3721: these passes are HBM-bound and small next to the
-// per-star kernels.
+// Work is split so that index arithmetic happens once per workgroup item,
+// never per element (64-bit division per element would dominate):
+// * conv_lines_kernel (n_inner > 1, the spatial PSF passes): item = one
+//   (outer, axis) line and a chunk of up to kLineChunk contiguous inner
+//   elements; threads run along the contiguous inner axis.
+// * conv_rows_kernel (n_inner == 1, the LSF along lambda): item = a segment
+//   of kSeg outputs of one row; the segment's inputs plus the kernel's halo
+//   are staged in LDS once and every output reads its taps from LDS.
+// The batched PSF (rbx_conv_batch) uses row items of the batch cube: one
+// spaxel row and a chunk of lambda, its galaxy found once per item.  Every
+// output sums taps in the same order t = 0 .. m-1 as the flat definition.
 #include "rbx_common.h"
 
 namespace rbx {
@@ -60,6 +66,136 @@ __global__ void conv2d_planes_kernel(const float* __restrict__ in, float* __rest
 
 constexpr int kDotThreads = 256;
 constexpr int kDotBlocksMax = 4096;
+constexpr int kLineChunk = 1024;  // inner elements per line item
+constexpr int kSeg = 2048;        // outputs per row item of the LSF
+constexpr int kHalo = 256;        // taps - 1 supported by the row kernel
+
+// One 1D convolution item of a line: out[o, x, i0 + k] for k < len.
+template <class Acc, class Tap>
+__device__ inline Acc line_tap_sum(const float* __restrict__ src, int x, int n_axis,
+                                   int64_t n_inner, const Tap* __restrict__ taps, int n_taps,
+                                   int start) {
+  Acc acc = 0;
+  for (int t = 0; t < n_taps; ++t) {
+    const int xs = x + start - t;
+    if (xs >= 0 && xs < n_axis) acc += Acc(taps[t]) * Acc(src[int64_t(xs) * n_inner]);
+  }
+  return acc;
+}
+
+// Items: (o, x, chunk) with chunk fastest.  Grid-strided.
+__global__ void __launch_bounds__(256) conv_lines_kernel(
+    const float* __restrict__ in, float* __restrict__ out, int64_t n_outer, int n_axis,
+    int64_t n_inner, const float* __restrict__ taps, int n_taps, int start, int64_t n_chunks,
+    int64_t n_items) {
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t c = item % n_chunks;
+    const int64_t ox = item / n_chunks;
+    const int x = int(ox % n_axis);
+    const int64_t o = ox / n_axis;
+    const int64_t i0 = c * kLineChunk;
+    const int64_t i1 = i0 + kLineChunk < n_inner ? i0 + kLineChunk : n_inner;
+    const float* base = in + o * n_axis * n_inner;
+    float* dst = out + (o * n_axis + x) * n_inner;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+      dst[i] = line_tap_sum<float>(base + i, x, n_axis, n_inner, taps, n_taps, start);
+  }
+}
+
+// Items: (row o, segment) of a row-contiguous convolution (n_inner == 1).
+// lds[k] = in[o, x0 + start - (m - 1) + k] (0 outside the row), so
+// out[o, x0 + v] = sum_t taps[t] lds[v + m - 1 - t].
+__global__ void __launch_bounds__(256) conv_rows_kernel(
+    const float* __restrict__ in, float* __restrict__ out, int n_axis,
+    const float* __restrict__ taps, int n_taps, int start, int64_t n_seg, int64_t n_items) {
+  __shared__ float lds[kSeg + kHalo];
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t o = item / n_seg;
+    const int x0 = int(item - o * n_seg) * kSeg;
+    const float* row = in + o * n_axis;
+    const int off = x0 + start - (n_taps - 1);
+    const int len = kSeg + n_taps - 1;
+    __syncthreads();
+    for (int k = threadIdx.x; k < len; k += blockDim.x) {
+      const int xs = off + k;
+      lds[k] = (xs >= 0 && xs < n_axis) ? row[xs] : 0.f;
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < kSeg && x0 + v < n_axis; v += blockDim.x) {
+      float acc = 0.f;
+      for (int t = 0; t < n_taps; ++t) {
+        const int xs = x0 + v + start - t;
+        if (xs >= 0 && xs < n_axis) acc += taps[t] * lds[v + n_taps - 1 - t];
+      }
+      out[o * n_axis + x0 + v] = acc;
+    }
+  }
+}
+
+// Fixed-order block reduction of one double per thread into partial[block].
+__device__ inline void block_partial(double acc, double* sm, double* __restrict__ partial) {
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kDotThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+// sum(a * conv(b)) per block, line items (n_inner > 1), double taps.
+__global__ void __launch_bounds__(kDotThreads) conv_dot_lines_kernel(
+    const float* __restrict__ a, const float* __restrict__ b, int64_t n_outer, int n_axis,
+    int64_t n_inner, const double* __restrict__ taps, int n_taps, int start, int64_t n_chunks,
+    int64_t n_items, double* __restrict__ partial) {
+  __shared__ double sm[kDotThreads];
+  double acc = 0.0;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t c = item % n_chunks;
+    const int64_t ox = item / n_chunks;
+    const int x = int(ox % n_axis);
+    const int64_t o = ox / n_axis;
+    const int64_t i0 = c * kLineChunk;
+    const int64_t i1 = i0 + kLineChunk < n_inner ? i0 + kLineChunk : n_inner;
+    const float* base = b + o * n_axis * n_inner;
+    const float* av = a + (o * n_axis + x) * n_inner;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+      acc += double(av[i]) * line_tap_sum<double>(base + i, x, n_axis, n_inner, taps, n_taps, start);
+  }
+  block_partial(acc, sm, partial);
+}
+
+// sum(a * conv(b)) per block, rows (n_inner == 1), double taps.
+__global__ void __launch_bounds__(kDotThreads) conv_dot_rows_kernel(
+    const float* __restrict__ a, const float* __restrict__ b, int n_axis,
+    const double* __restrict__ taps, int n_taps, int start, int64_t n_seg, int64_t n_items,
+    double* __restrict__ partial) {
+  __shared__ float lds[kSeg + kHalo];
+  __shared__ double sm[kDotThreads];
+  double acc = 0.0;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t o = item / n_seg;
+    const int x0 = int(item - o * n_seg) * kSeg;
+    const float* row = b + o * n_axis;
+    const int off = x0 + start - (n_taps - 1);
+    const int len = kSeg + n_taps - 1;
+    __syncthreads();
+    for (int k = threadIdx.x; k < len; k += blockDim.x) {
+      const int xs = off + k;
+      lds[k] = (xs >= 0 && xs < n_axis) ? row[xs] : 0.f;
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < kSeg && x0 + v < n_axis; v += blockDim.x) {
+      double c = 0.0;
+      for (int t = 0; t < n_taps; ++t) {
+        const int xs = x0 + v + start - t;
+        if (xs >= 0 && xs < n_axis) c += taps[t] * double(lds[v + n_taps - 1 - t]);
+      }
+      acc += double(a[o * n_axis + x0 + v]) * c;
+    }
+  }
+  block_partial(acc, sm, partial);
+}
 
 // Per-block partial of sum(a * conv_axis(b)) in double, fixed tree order.
 // The sigma-derivative taps sum to zero, so conv(dtaps, b) cancels strongly:
@@ -154,48 +290,53 @@ __device__ inline BatchPos batch_pos(const rbx_galaxy* __restrict__ gal, int n_g
   return b;
 }
 
-__global__ void conv_batch_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                  const rbx_galaxy* __restrict__ gal, int n_gal, int n_rows,
-                                  int nt, int axis, const float* __restrict__ taps, int n_taps,
-                                  int start) {
-  const int64_t total = int64_t(n_rows) * nt;
-  for (int64_t idx = gtid(); idx < total; idx += int64_t(gridDim.x) * blockDim.x) {
-    const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, idx);
-    float acc = 0.f;
-    if (b.ok) {
-      for (int t = 0; t < n_taps; ++t) {
-        const int us = b.u + start - t;
-        if (us >= 0 && us < b.nb) acc += taps[t] * in[b.base + us * b.stride];
+// Row items: (spaxel row p of the batch cube, chunk of kLineChunk lambda);
+// the galaxy and the row's (y, x) are found once per item.
+__global__ void __launch_bounds__(256) conv_batch_kernel(
+    const float* __restrict__ in, float* __restrict__ out, const rbx_galaxy* __restrict__ gal,
+    int n_gal, int n_rows, int nt, int axis, const float* __restrict__ taps, int n_taps,
+    int start, int n_chunks, int64_t n_items) {
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int p = int(item / n_chunks);
+    const int l0 = int(item - int64_t(p) * n_chunks) * kLineChunk;
+    const int l1 = l0 + kLineChunk < nt ? l0 + kLineChunk : nt;
+    const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, int64_t(p) * nt);
+    for (int l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+      float acc = 0.f;
+      if (b.ok) {
+        for (int t = 0; t < n_taps; ++t) {
+          const int us = b.u + start - t;
+          if (us >= 0 && us < b.nb) acc += taps[t] * in[b.base + l + us * b.stride];
+        }
       }
+      out[int64_t(p) * nt + l] = acc;
     }
-    out[idx] = acc;
   }
 }
 
 __global__ void __launch_bounds__(kDotThreads) conv_batch_dot_partial_kernel(
     const float* __restrict__ a, const float* __restrict__ bsrc,
     const rbx_galaxy* __restrict__ gal, int n_gal, int n_rows, int nt, int axis,
-    const double* __restrict__ taps, int n_taps, int start, double* __restrict__ partial) {
+    const double* __restrict__ taps, int n_taps, int start, int n_chunks, int64_t n_items,
+    double* __restrict__ partial) {
   __shared__ double sm[kDotThreads];
-  const int64_t total = int64_t(n_rows) * nt;
   double acc = 0.0;
-  for (int64_t idx = gtid(); idx < total; idx += int64_t(gridDim.x) * blockDim.x) {
-    const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, idx);
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int p = int(item / n_chunks);
+    const int l0 = int(item - int64_t(p) * n_chunks) * kLineChunk;
+    const int l1 = l0 + kLineChunk < nt ? l0 + kLineChunk : nt;
+    const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, int64_t(p) * nt);
     if (!b.ok) continue;
-    double c = 0.0;
-    for (int t = 0; t < n_taps; ++t) {
-      const int us = b.u + start - t;
-      if (us >= 0 && us < b.nb) c += taps[t] * double(bsrc[b.base + us * b.stride]);
+    for (int l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+      double c = 0.0;
+      for (int t = 0; t < n_taps; ++t) {
+        const int us = b.u + start - t;
+        if (us >= 0 && us < b.nb) c += taps[t] * double(bsrc[b.base + l + us * b.stride]);
+      }
+      acc += double(a[int64_t(p) * nt + l]) * c;
     }
-    acc += double(a[idx]) * c;
   }
-  sm[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = kDotThreads / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+  block_partial(acc, sm, partial);
 }
 
 static int grid_for(int64_t n, int threads, int64_t cap) {
@@ -207,12 +348,26 @@ static int grid_for(int64_t n, int threads, int64_t cap) {
 
 static int dot_blocks(int64_t total) { return grid_for(total, kDotThreads * 8, kDotBlocksMax); }
 
+static int64_t item_blocks(int64_t items, int64_t cap) { return items < cap ? (items > 0 ? items : 1) : cap; }
+
 int conv_axis(const float* in, float* out, int64_t n_outer, int n_axis, int64_t n_inner,
               const float* taps, int n_taps, int start, hipStream_t s) {
   const int64_t total = n_outer * n_axis * n_inner;
   if (total == 0) return RBX_OK;
-  hipLaunchKernelGGL(conv_axis_kernel, dim3(grid_for(total, 256, 1 << 20)), dim3(256), 0, s, in,
-                     out, n_outer, n_axis, n_inner, taps, n_taps, start);
+  if (n_inner == 1 && n_taps - 1 <= kHalo) {
+    const int64_t n_seg = (n_axis + kSeg - 1) / kSeg;
+    const int64_t items = n_outer * n_seg;
+    hipLaunchKernelGGL(conv_rows_kernel, dim3(item_blocks(items, 1 << 20)), dim3(256), 0, s, in,
+                       out, n_axis, taps, n_taps, start, n_seg, items);
+  } else if (n_inner > 1) {
+    const int64_t n_chunks = (n_inner + kLineChunk - 1) / kLineChunk;
+    const int64_t items = n_outer * n_axis * n_chunks;
+    hipLaunchKernelGGL(conv_lines_kernel, dim3(item_blocks(items, 1 << 20)), dim3(256), 0, s, in,
+                       out, n_outer, n_axis, n_inner, taps, n_taps, start, n_chunks, items);
+  } else {
+    hipLaunchKernelGGL(conv_axis_kernel, dim3(grid_for(total, 256, 1 << 20)), dim3(256), 0, s, in,
+                       out, n_outer, n_axis, n_inner, taps, n_taps, start);
+  }
   return check_launch("rbx_conv_axis");
 }
 
@@ -272,8 +427,19 @@ int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer, int32_t n
     return fail(RBX_EWORKSPACE, "rbx_conv_axis_dot: workspace too small");
   hipStream_t s = as_stream(stream);
   double* partial = static_cast<double*>(workspace);
-  hipLaunchKernelGGL(conv_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,
-                     n_axis, n_inner, taps, n_taps, start, partial);
+  if (n_inner == 1 && n_taps - 1 <= kHalo) {
+    const int64_t n_seg = (n_axis + kSeg - 1) / kSeg;
+    hipLaunchKernelGGL(conv_dot_rows_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_axis,
+                       taps, n_taps, start, n_seg, n_outer * n_seg, partial);
+  } else if (n_inner > 1) {
+    const int64_t n_chunks = (n_inner + kLineChunk - 1) / kLineChunk;
+    hipLaunchKernelGGL(conv_dot_lines_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,
+                       n_axis, n_inner, taps, n_taps, start, n_chunks, n_outer * n_axis * n_chunks,
+                       partial);
+  } else {
+    hipLaunchKernelGGL(conv_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,
+                       n_axis, n_inner, taps, n_taps, start, partial);
+  }
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, nblk, result);
   return check_launch("rbx_conv_axis_dot");
 }
@@ -288,9 +454,11 @@ int rbx_conv_batch(const float* in, float* out, const rbx_galaxy* galaxies, int3
   if (in == out) return fail(RBX_EINVAL, "rbx_conv_batch: in-place convolution not supported");
   const int64_t total = int64_t(n_rows) * n_target;
   if (total == 0) return RBX_OK;
-  hipLaunchKernelGGL(conv_batch_kernel, dim3(grid_for(total, 256, 1 << 20)), dim3(256), 0,
+  const int n_chunks = (n_target + kLineChunk - 1) / kLineChunk;
+  const int64_t items = int64_t(n_rows) * n_chunks;
+  hipLaunchKernelGGL(conv_batch_kernel, dim3(item_blocks(items, 1 << 20)), dim3(256), 0,
                      as_stream(stream), in, out, galaxies, n_gal, n_rows, n_target, axis, taps,
-                     n_taps, start);
+                     n_taps, start, n_chunks, items);
   return check_launch("rbx_conv_batch");
 }
 
@@ -308,8 +476,10 @@ int rbx_conv_batch_dot(const float* a, const float* b, const rbx_galaxy* galaxie
     return fail(RBX_EWORKSPACE, "rbx_conv_batch_dot: workspace too small");
   hipStream_t s = as_stream(stream);
   double* partial = static_cast<double*>(workspace);
+  const int n_chunks = (n_target + kLineChunk - 1) / kLineChunk;
   hipLaunchKernelGGL(conv_batch_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b,
-                     galaxies, n_gal, n_rows, n_target, axis, taps, n_taps, start, partial);
+                     galaxies, n_gal, n_rows, n_target, axis, taps, n_taps, start, n_chunks,
+                     int64_t(n_rows) * n_chunks, partial);
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, nblk, out);
   return check_launch("rbx_conv_batch_dot");
 }

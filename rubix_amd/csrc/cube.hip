@@ -248,6 +248,87 @@ __device__ inline float star_value(const Rows& R, const StarState& s, const Knot
   return S0 + K.theta * (S1 - S0);
 }
 
+// ------------------------------------------------------------ LDS windows
+//
+// A (star, lambda slab) item touches only the knots between the segments of
+// the slab's first and last bins (targets ascending).  The window's
+// unshifted knots and the star's combined template F_k = sum_c w_c f_c[k]
+// are loaded once with coalesced loads into the wave's LDS; the bins then
+// interpolate from LDS instead of gathering four template pairs each
+// through the L1.
+constexpr int kWindow = 640;  // knots per staged window (LDS floats per array)
+
+// Order a wave's LDS stores before its later loads (the wave's own data;
+// LDS instructions of one wave complete in order).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Segment of target x inside a staged window of knots [k_lo, k_lo + nk):
+// the locator guess clamped into the window, then the exact correction on
+// the window's knots (for ascending targets the true segment lies inside).
+struct WinKnots {
+  int li;       // local left knot, in [0, nk - 2]
+  float theta;  // as Knots::theta
+  float kappa;  // as Knots::kappa (only when requested)
+};
+
+template <bool kKappa>
+__device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, int k_lo, int nk,
+                                     const float* lam) {
+  const int n = g.n_wave;
+  float fm = fmaf(x, h.c1, h.c0);
+  fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
+  // compact locator copy (4 bytes per cell): k0 = knots <= cell start - 1
+  const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
+  const float y = x * h.inv_a;
+  int li = k0 - k_lo;
+  li = li < 0 ? 0 : (li > nk - 2 ? nk - 2 : li);
+  if (li + 1 < nk - 1 && lam[li + 1] <= y) ++li;
+  float l0 = lam[li], l1 = lam[li + 1];
+  float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
+  // the global segment [li + k_lo, li + k_lo + 1] moves within [0, n - 1]
+  const int li_max = nk - 2, li_min = 0;
+  bool step = (x >= wr && li < li_max && li + k_lo + 1 < n - 1) || (x < wl && li > li_min && li + k_lo > 0);
+  if (__ballot(step)) {
+    while (step) {
+      li += x >= wr ? 1 : -1;
+      l0 = lam[li];
+      l1 = lam[li + 1];
+      wl = mul_nocontract(l0, h.a);
+      wr = mul_nocontract(l1, h.a);
+      step = (x >= wr && li < li_max && li + k_lo + 1 < n - 1) || (x < wl && li > li_min && li + k_lo > 0);
+    }
+  }
+  const int i = li + k_lo + 1;  // global right knot
+  const float dx = wr - wl;
+  const float r = __builtin_amdgcn_rcpf(dx);
+  const bool flat = !(dx > 0.f);
+  const float th = flat ? 0.f : (x - wl) * r;
+  const bool left = (i == 1) && (x < wl);
+  const bool right = (i == n - 1) && (x > wr);
+  WinKnots K;
+  K.li = li;
+  K.theta = left ? 0.f : (right ? 1.f : th);
+  K.kappa = 0.f;
+  if (kKappa) K.kappa = (left || right || flat) ? 0.f : (l0 + th * (l1 - l0)) * r;
+  return K;
+}
+
+// Knot window [k_lo, k_lo + nk) holding every segment of the bins [j0, j1)
+// (ascending targets): from the first bin's left knot to the last bin's
+// right knot.
+__device__ inline void slab_window(const SspView& g, const rbx_target& tg, uint32_t j0,
+                                   uint32_t j1, const Shift& h, int* k_lo, int* nk) {
+  const float xf = ld_at(tg.wave, j0), xl = ld_at(tg.wave, j1 - 1);
+  const int lo = int(knots_of<false>(g, xf, h).i0);
+  const int hi = int(knots_of<false>(g, xl, h).i0) + 1;
+  *k_lo = __builtin_amdgcn_readfirstlane(lo);
+  *nk = __builtin_amdgcn_readfirstlane(hi - lo + 1);
+}
+
 // Work items of the slab-mapped per-star kernels: item q is lambda slab
 // q % 8 of the W stars [(q / 8) W, (q / 8) W + W), one per wavefront of a
 // workgroup of W waves.  Workgroup b takes items b, b + G, b + 2G, ... with
@@ -278,21 +359,47 @@ __global__ void __launch_bounds__(64 * W) star_norm_kernel(SspView g, rbx_target
                                                            const StarState* __restrict__ state,
                                                            int slab_bins,
                                                            float* __restrict__ nn_part) {
+  __shared__ float lam_s[W][kWindow];
+  __shared__ float F_s[W][kWindow];
   for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
     const SlabTask t = slab_task<W>(q);
     if (t.r >= n) continue;
     const StarState s = state[t.r];
     float acc = 0.f;
-    if (s.row >= 0) {
+    const uint32_t j0 = uint32_t(t.slab * slab_bins);
+    const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
+    if (s.row >= 0 && j0 < j1) {
       const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
       const Shift h = shift_of(g, s.a, s.inv_a);
-      const uint32_t j0 = uint32_t(t.slab * slab_bins);
-      const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
+      int k_lo = 0, nk = kWindow + 1;
+      if (tg.ascending) slab_window(g, tg, j0, j1, h, &k_lo, &nk);
+      if (nk <= kWindow) {
+        float* lam = lam_s[t.wave];
+        float* F = F_s[t.wave];
+        for (int k = t.lane; k < nk; k += 64) {
+          const uint32_t kk = uint32_t(k_lo + k);
+          lam[k] = ld_at(g.wave, kk);
+          float f = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
+          F[k] = f;
+        }
+        wave_lds_sync();
 #pragma unroll 2
-      for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
-        const Pair xd = ld_pair(tg.pairs, 2 * j);
-        const Knots K = knots_of<false>(g, xd.lo, h);
-        acc += star_value(R, s, K) * xd.hi;
+        for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
+          const Pair xd = ld_pair(tg.pairs, 2 * j);
+          const WinKnots K = win_knots<false>(g, xd.lo, h, k_lo, nk, lam);
+          const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
+          acc += (S0 + K.theta * (S1 - S0)) * xd.hi;
+        }
+        wave_lds_sync();
+      } else {
+#pragma unroll 2
+        for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
+          const Pair xd = ld_pair(tg.pairs, 2 * j);
+          const Knots K = knots_of<false>(g, xd.lo, h);
+          acc += star_value(R, s, K) * xd.hi;
+        }
       }
       acc = wave_sum(acc);
     }
@@ -381,6 +488,88 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
   }
 }
 
+// Staged variant for ascending targets with at most kLaneBins bins per lane
+// per slab: item q = lambda slab q % 8 of W consecutive spaxel chunks (one
+// per wavefront).  Each lane keeps its bins' wavelengths and f64
+// accumulators in registers across the chunk's stars; per star the slab's
+// knot window is staged in LDS once (as in star_norm_kernel) and every bin
+// interpolates from it.  Same arithmetic per (star, bin) as
+// cube_accum_kernel and the same star order per bin, so the partials are
+// bitwise those of the per-tile kernel.
+constexpr int kLaneBins = 8;
+
+template <int W>
+__global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
+    SspView g, rbx_target tg, const StarState* __restrict__ state,
+    const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr, int slab_bins,
+    int64_t n_items, double* __restrict__ partial) {
+  __shared__ float lam_s[W][kWindow];
+  __shared__ float F_s[W][kWindow];
+  const int n_chunks = *n_chunks_ptr;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  float* lam = lam_s[wave];
+  float* F = F_s[wave];
+  for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
+    const int slab = int(q & 7);
+    const int64_t chunk = (q >> 3) * W + wave;
+    if (chunk >= n_chunks) continue;
+    const uint32_t j0 = uint32_t(slab * slab_bins);
+    const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
+    if (j0 >= j1) continue;
+    const ChunkDesc d = desc[chunk];
+    float x[kLaneBins];
+    double acc[kLaneBins];
+#pragma unroll
+    for (int t = 0; t < kLaneBins; ++t) {
+      const uint32_t j = j0 + lane + 64u * t;
+      x[t] = ld_at(tg.wave, j < j1 ? j : j1 - 1);
+      acc[t] = 0.0;
+    }
+    for (int r = d.begin; r < d.end; ++r) {
+      const StarState s = state[r];
+      if (s.row < 0) continue;
+      const Rows R = rows_of(g, s.row);
+      const Shift h = shift_of(g, s.a, s.inv_a);
+      int k_lo, nk;
+      slab_window(g, tg, j0, j1, h, &k_lo, &nk);
+      if (nk <= kWindow) {
+        for (int k = lane; k < nk; k += 64) {
+          const uint32_t kk = uint32_t(k_lo + k);
+          lam[k] = ld_at(g.wave, kk);
+          float f = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
+          F[k] = f;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int t = 0; t < kLaneBins; ++t) {
+          if (j0 + lane + 64u * t < j1) {
+            const WinKnots K = win_knots<false>(g, x[t], h, k_lo, nk, lam);
+            const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
+            acc[t] += double((S0 + K.theta * (S1 - S0)) * s.sf);
+          }
+        }
+        wave_lds_sync();
+      } else {
+#pragma unroll
+        for (int t = 0; t < kLaneBins; ++t) {
+          if (j0 + lane + 64u * t < j1) {
+            const Knots K = knots_of<false>(g, x[t], h);
+            acc[t] += double(star_value(R, s, K) * s.sf);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kLaneBins; ++t) {
+      const uint32_t j = j0 + lane + 64u * t;
+      if (j < j1) partial[chunk * tg.n + j] = acc[t];
+    }
+  }
+}
+
 __global__ void cube_combine_kernel(const double* __restrict__ partial,
                                     const int32_t* __restrict__ chunk_off, int32_t n_spaxel,
                                     int32_t nt, float* __restrict__ cube) {
@@ -406,10 +595,12 @@ constexpr int kRedStride = 17;  // padded LDS row of one lane's 16 sums
 
 // v[0..15] of all 64 lanes -> lane 4q holds the wave total of v[q] (double).
 // The order is fixed: 16 consecutive lanes in order, then pairs of those.
+// buf is the wave's own LDS (64 * kRedStride floats).
 __device__ inline double lds_sum16(const float v[16], float* buf, int lane) {
+  wave_lds_sync();
 #pragma unroll
   for (int q = 0; q < 16; ++q) buf[lane * kRedStride + q] = v[q];
-  __syncthreads();
+  wave_lds_sync();
   const int q = lane >> 2, p = lane & 3;
   double acc = 0.0;
 #pragma unroll
@@ -420,8 +611,10 @@ __device__ inline double lds_sum16(const float v[16], float* buf, int lane) {
 }
 
 // Slab-mapped like star_norm_kernel; writes the 16 sums of (star, slab) to
-// vpart (value sums, float) / apart (Doppler sums, double).  The item loop is
-// uniform over the workgroup (lds_sum16 synchronises it).
+// vpart (value sums, float) / apart (Doppler sums, double).  The four corner
+// rows are gathered per bin (8-byte pairs from a scalar base): staging them
+// in LDS as the forward does needs 5 windows per wave, and the occupancy
+// that costs made the adjoint slower.
 template <int W>
 __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
     SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
@@ -429,46 +622,46 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
     double* __restrict__ apart) {
   __shared__ float red[W][64 * kRedStride];
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-  const SlabTask t = slab_task<W>(item);
-  const bool live = t.r < n;
-  float v[16];
+    const SlabTask t = slab_task<W>(item);
+    const bool live = t.r < n;
+    float v[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = 0.f;
-  StarState s;
-  s.row = -1;
-  if (live) s = state[t.r];
-  if (s.row >= 0) {
-    const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
-    const float* grow = dcube + int64_t(s.spaxel) * tg.n;
-    const Shift h = shift_of(g, s.a, s.inv_a);
-    const uint32_t j0 = uint32_t(t.slab * slab_bins);
-    const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
+    for (int q = 0; q < 16; ++q) v[q] = 0.f;
+    StarState s;
+    s.row = -1;
+    if (live) s = state[t.r];
+    if (s.row >= 0) {
+      const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
+      const float* grow = dcube + int64_t(s.spaxel) * tg.n;
+      const Shift h = shift_of(g, s.a, s.inv_a);
+      const uint32_t j0 = uint32_t(t.slab * slab_bins);
+      const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
 #pragma unroll 2
-    for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
-      const Pair xd = ld_pair(tg.pairs, 2 * j);
-      const float gj = ld_at(grow, j), dj = xd.hi;
-      const Knots K = knots_of<true>(g, xd.lo, h);
-      const float gk = -gj * K.kappa, dk = -dj * K.kappa;
+      for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
+        const Pair xd = ld_pair(tg.pairs, 2 * j);
+        const float gj = ld_at(grow, j), dj = xd.hi;
+        const Knots K = knots_of<true>(g, xd.lo, h);
+        const float gk = -gj * K.kappa, dk = -dj * K.kappa;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const Pair f = ld_pair(R.f[c], K.i0);
-        const float f0 = f.lo, f1 = f.hi;
-        const float df = f1 - f0;
-        const float Rc = f0 + K.theta * df;
-        v[c] += gj * Rc;
-        v[4 + c] += dj * Rc;
-        v[8 + c] += gk * df;
-        v[12 + c] += dk * df;
+        for (int c = 0; c < 4; ++c) {
+          const Pair f = ld_pair(R.f[c], K.i0);
+          const float f0 = f.lo, f1 = f.hi;
+          const float df = f1 - f0;
+          const float Rc = f0 + K.theta * df;
+          v[c] += gj * Rc;
+          v[4 + c] += dj * Rc;
+          v[8 + c] += gk * df;
+          v[12 + c] += dk * df;
+        }
       }
     }
-  }
-  const double tot = lds_sum16(v, red[t.wave], t.lane);
-  const int q = t.lane >> 2;
-  if (live && (t.lane & 3) == 0) {
-    const int64_t base = (t.r * kSlabs + t.slab) * 8;
-    if (q < 8) vpart[base + q] = float(tot);
-    else apart[base + q - 8] = tot;
-  }
+    const double tot = lds_sum16(v, red[t.wave], t.lane);
+    const int q = t.lane >> 2;
+    if (live && (t.lane & 3) == 0) {
+      const int64_t base = (t.r * kSlabs + t.slab) * 8;
+      if (q < 8) vpart[base + q] = float(tot);
+      else apart[base + q - 8] = tot;
+    }
   }
 }
 
@@ -587,6 +780,7 @@ __global__ void ssp_locate_kernel(const float* __restrict__ wave, int nw, float 
     }
     const int b = k < nw - 2 ? k : nw - 2;
     loc[4 * m] = k;
+    loc[4 * int64_t(nloc) + m] = k;  // compact copy
     for (int q = 0; q < 3; ++q) {
       const int kk = b + q;
       const float v = kk < 0 ? -INFINITY : (kk >= nw ? INFINITY : wave[kk]);
@@ -829,7 +1023,13 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   rc = build_chunks(L.offsets, ns, t.chunk, L.chunk_off, L.scan_blocks, L.desc, s);
   if (rc) return rc;
   const int tpx = (n_tiles + 7) / 8;
-  if (n > 0) {
+  if (n > 0 && target->ascending && tpx <= kLaneBins) {
+    const int64_t items = int64_t(8) * ((L.max_chunks + 3) / 4);
+    const int64_t grid = item_grid(items, t.persist, 256);
+    hipLaunchKernelGGL(cube_accum_staged_kernel<4>, dim3(grid), dim3(256), 0, s, g, *target,
+                       L.state, L.desc, L.chunk_off + ns, tpx * kTile, items, L.partial);
+    if ((rc = check_launch(who))) return rc;
+  } else if (n > 0) {
     const int64_t items = int64_t(8) * tpx * L.max_chunks;
     if (items >= (int64_t(1) << 34))
       return failf(RBX_EINVAL, "%s: problem too large for one launch", who);

@@ -195,7 +195,7 @@ class PreparedSSP:
         if span * self.locate_inv_h + 2 > nloc - 1:
             self.n_locate = nloc = int(math.floor(span * self.locate_inv_h)) + 3
         self.cum = torch.empty(nm * na * nw, dtype=torch.float64, device=dev)
-        self.locate = torch.empty(4 * nloc, dtype=torch.int32, device=dev)
+        self.locate = torch.empty(5 * nloc, dtype=torch.int32, device=dev)
         self.shape = (nm, na, nw)
         self._struct = self._make_struct()
         call("rbx_ssp_prepare", self._struct, ptr(self.cum), ptr(self.locate), stream_handle(dev))

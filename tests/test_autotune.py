@@ -13,11 +13,11 @@ def test_descent_finds_separable_optimum_and_caches(tmp_path, monkeypatch):
 
     def cost(run, t, reps=3):
         calls.append(t)
-        return abs(t.waves - 2) + 0.5 * abs(t.persist - 4) + abs(t.chunk - 512) / 256
+        return abs(t.waves - 2) + 0.5 * abs(t.persist - 16) + abs(t.chunk - 512) / 256
 
     monkeypatch.setattr(A, "time_ms", cost)
     best, seen = A.autotune(None, "key-a")
-    assert best == Tuning(chunk=512, waves=2, persist=4)
+    assert best == Tuning(chunk=512, waves=2, persist=16)
     # defaults + 3 other waves + 3 other persists + 4 other chunks, each timed once
     assert len(calls) == len(set(calls)) == 11
     data = json.loads(cache.read_text())
