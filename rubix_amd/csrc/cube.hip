@@ -25,13 +25,13 @@
 //   The slab-mapped kernels put lambda slab b % 8 on XCD b % 8, so each
 //   XCD's L2 holds only its slab of the templates.
 //   Adjoint: star_bwd (wavefront / (star, lambda slab)) gathers
-//   dL/dcube[spaxel_i, :] over the slab's bins and accumulates, per template
-//   corner c, V_c = sum_j g_j R_c(j), A_c = sum_j g_j dR_c(j)/da and the same
-//   with the weights dtarget_j, where R_c(j) is template row c interpolated
-//   at target_j; star_grad adds the slabs in order and forms every per-star
-//   gradient (mass, age, metallicity, v_los) in closed form from these 16
-//   sums, T, Nn and the prefix-sum window.  Spectra are recomputed, never
-//   stored.
+//   dL/dcube[spaxel_i, :] over the slab's bins and accumulates seven sums of
+//   the combined template rows R_w (bilinear weights), R_u, R_s (their
+//   derivatives in log Z, log t) interpolated at target_j and of dR_w/da,
+//   weighted by g_j and dtarget_j; star_grad adds the slabs in order and
+//   forms every per-star gradient (mass, age, metallicity, v_los) in closed
+//   form from these sums, T, Nn and the prefix-sum window.  Spectra are
+//   recomputed, never stored.
 #include <float.h>
 #include <math.h>
 
@@ -78,7 +78,7 @@ __device__ inline void window_sums(const SspView& g, int row, int kl, int kh, do
 __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
                                  const int32_t* __restrict__ perm,
                                  const int32_t* __restrict__ skey,
-                                 StarState* __restrict__ out) {
+                                 StarState* __restrict__ out, float* __restrict__ gradw) {
   const int64_t n = st.n;
   for (int64_t r = gtid(); r < n; r += int64_t(gridDim.x) * blockDim.x) {
     const int32_t i = perm[r];
@@ -97,6 +97,11 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
     const bool keep = c.inside && key >= 0 && key < st.n_spaxel;
     s.row = keep ? c.row : -1;
     for (int q = 0; q < 4; ++q) s.w[q] = c.w[q];
+    // d w_c / d(log Z) and d w_c / d(log t) for the adjoint's combined rows
+    for (int q = 0; q < 4; ++q) {
+      gradw[r * 8 + q] = c.gu[q];
+      gradw[r * 8 + 4 + q] = c.gs[q];
+    }
     // in-range window of the mask (w >= min target) & (w <= max target)
     const int klo_le = locate(g, tg.wmin, a, s.inv_a);
     int klo = (klo_le >= 0 && knot(g, klo_le, a) == tg.wmin) ? klo_le : klo_le + 1;
@@ -586,47 +591,59 @@ __global__ void cube_combine_kernel(const double* __restrict__ partial,
 
 // ------------------------------------------------------------ adjoint
 
-// Per-slab sums of one star (see the file comment): V = sum_j w_j R_c(j) and
-// A = sum_j w_j dR_c(j)/da for the weights w = dL/dC (g) and dtarget (d).
+// Per-slab sums of one star: value sums of interpolated template rows and
+// Doppler sums of their a-derivatives, weighted by dL/dC (g) and dtarget (d).
 // Each lane accumulates its bins in float32; the wave's 64 lane sums are
-// added in double in a fixed order (lds_sum16), so the Doppler sums A, which
+// added in double in a fixed order (lds_sum), so the Doppler sums, which
 // cancel strongly across bins, lose nothing beyond the float32 per-bin terms.
-constexpr int kRedStride = 17;  // padded LDS row of one lane's 16 sums
+constexpr int kRedStride = 17;  // padded LDS row of one lane's (up to 16) sums
 
-// v[0..15] of all 64 lanes -> lane 4q holds the wave total of v[q] (double).
-// The order is fixed: 16 consecutive lanes in order, then pairs of those.
-// buf is the wave's own LDS (64 * kRedStride floats).
-__device__ inline double lds_sum16(const float v[16], float* buf, int lane) {
+// v[0..K) of all 64 lanes -> lane 4q (q < K) holds the wave total of v[q]
+// (double).  The order is fixed: 16 consecutive lanes in order, then pairs
+// of those.  buf is the wave's own LDS (64 * kRedStride floats).
+template <int K>
+__device__ inline double lds_sum(const float v[K], float* buf, int lane) {
+  static_assert(K <= 16, "at most 16 values");
   wave_lds_sync();
 #pragma unroll
-  for (int q = 0; q < 16; ++q) buf[lane * kRedStride + q] = v[q];
+  for (int q = 0; q < K; ++q) buf[lane * kRedStride + q] = v[q];
   wave_lds_sync();
   const int q = lane >> 2, p = lane & 3;
   double acc = 0.0;
+  if (q < K) {
 #pragma unroll
-  for (int t = 0; t < 16; ++t) acc += double(buf[(p * 16 + t) * kRedStride + q]);
+    for (int t = 0; t < 16; ++t) acc += double(buf[(p * 16 + t) * kRedStride + q]);
+  }
   acc += __shfl_xor(acc, 1, 64);
   acc += __shfl_xor(acc, 2, 64);
   return acc;
 }
 
-// Slab-mapped like star_norm_kernel; writes the 16 sums of (star, slab) to
-// vpart (value sums, float) / apart (Doppler sums, double).  The four corner
-// rows are gathered per bin (8-byte pairs from a scalar base): staging them
-// in LDS as the forward does needs 5 windows per wave, and the occupancy
-// that costs made the adjoint slower.
+// Slab-mapped like star_norm_kernel.  The per-star gradients need the four
+// corner rows only through three weightings: the bilinear weights w_c and
+// their derivatives gu_c = dw_c/d(log Z), gs_c = dw_c/d(log t).  Per bin the
+// kernel gathers the four corner pairs (8-byte loads from scalar bases),
+// forms the combined rows R_w, R_u, R_s at the bin and accumulates the seven
+// sums star_grad_kernel uses:
+//   Vg_w, Vg_u, Vg_s = sum_j g_j R_{w,u,s}(j),  Vd_u, Vd_s = sum_j d_j R_{u,s}(j),
+//   Ag_w, Ad_w = sum_j {g_j, d_j} dR_w(j)/da      (d = dtarget),
+// float32 per lane, added across lanes in double in a fixed order; vpart
+// gets the five value sums, apart the two Doppler sums.  Combining the
+// corners per knot takes differences of neighbouring templates while they
+// are small and nearly exact, before any sum over bins.  (Staging the rows
+// in LDS as the forward kernels do cost more occupancy than it saved here.)
 template <int W>
 __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
     SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
-    const float* __restrict__ dcube, int slab_bins, float* __restrict__ vpart,
-    double* __restrict__ apart) {
+    const float* __restrict__ gradw, const float* __restrict__ dcube, int slab_bins,
+    float* __restrict__ vpart, double* __restrict__ apart) {
   __shared__ float red[W][64 * kRedStride];
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const SlabTask t = slab_task<W>(item);
     const bool live = t.r < n;
-    float v[16];
+    float v[7];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = 0.f;
+    for (int q = 0; q < 7; ++q) v[q] = 0.f;
     StarState s;
     s.row = -1;
     if (live) s = state[t.r];
@@ -634,6 +651,12 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
       const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
       const float* grow = dcube + int64_t(s.spaxel) * tg.n;
       const Shift h = shift_of(g, s.a, s.inv_a);
+      float gu[4], gs[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        gu[c] = gradw[t.r * 8 + c];
+        gs[c] = gradw[t.r * 8 + 4 + c];
+      }
       const uint32_t j0 = uint32_t(t.slab * slab_bins);
       const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
 #pragma unroll 2
@@ -642,25 +665,36 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
         const float gj = ld_at(grow, j), dj = xd.hi;
         const Knots K = knots_of<true>(g, xd.lo, h);
         const float gk = -gj * K.kappa, dk = -dj * K.kappa;
+        float w0 = 0.f, w1 = 0.f, u0 = 0.f, u1 = 0.f, s0 = 0.f, s1 = 0.f;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const Pair f = ld_pair(R.f[c], K.i0);
-          const float f0 = f.lo, f1 = f.hi;
-          const float df = f1 - f0;
-          const float Rc = f0 + K.theta * df;
-          v[c] += gj * Rc;
-          v[4 + c] += dj * Rc;
-          v[8 + c] += gk * df;
-          v[12 + c] += dk * df;
+          w0 += s.w[c] * f.lo;
+          w1 += s.w[c] * f.hi;
+          u0 += gu[c] * f.lo;
+          u1 += gu[c] * f.hi;
+          s0 += gs[c] * f.lo;
+          s1 += gs[c] * f.hi;
         }
+        const float dw = w1 - w0;
+        const float Rw = w0 + K.theta * dw;
+        const float Ru = u0 + K.theta * (u1 - u0);
+        const float Rs = s0 + K.theta * (s1 - s0);
+        v[0] += gj * Rw;
+        v[1] += gj * Ru;
+        v[2] += gj * Rs;
+        v[3] += dj * Ru;
+        v[4] += dj * Rs;
+        v[5] += gk * dw;
+        v[6] += dk * dw;
       }
     }
-    const double tot = lds_sum16(v, red[t.wave], t.lane);
+    const double tot = lds_sum<7>(v, red[t.wave], t.lane);
     const int q = t.lane >> 2;
-    if (live && (t.lane & 3) == 0) {
+    if (live && (t.lane & 3) == 0 && q < 7) {
       const int64_t base = (t.r * kSlabs + t.slab) * 8;
-      if (q < 8) vpart[base + q] = float(tot);
-      else apart[base + q - 8] = tot;
+      if (q < 5) vpart[base + q] = float(tot);
+      else apart[base + q - 5] = tot;
     }
   }
 }
@@ -685,33 +719,27 @@ __global__ void __launch_bounds__(256) star_grad_kernel(SspView g, rbx_stars st,
       if (dvlos) dvlos[i] = 0.f;
       continue;
     }
-    double Vg[4] = {0, 0, 0, 0}, Vd[4] = {0, 0, 0, 0}, Ag[4] = {0, 0, 0, 0}, Ad[4] = {0, 0, 0, 0};
+    // slab sums in slab order: Vg_w, Vg_u, Vg_s, Vd_u, Vd_s | Ag_w, Ad_w
+    double Pg_m = 0, Pg_u = 0, Pg_s = 0, Pd_u = 0, Pd_s = 0, Pg_a = 0, Pd_a = 0;
     for (int q = 0; q < kSlabs; ++q) {
       const float* vp = vpart + (r * kSlabs + q) * 8;
       const double* ap = apart + (r * kSlabs + q) * 8;
-      for (int c = 0; c < 4; ++c) {
-        Vg[c] += vp[c];
-        Vd[c] += vp[4 + c];
-        Ag[c] += ap[c];
-        Ad[c] += ap[4 + c];
-      }
+      Pg_m += vp[0];
+      Pg_u += vp[1];
+      Pg_s += vp[2];
+      Pd_u += vp[3];
+      Pd_s += vp[4];
+      Pg_a += ap[0];
+      Pd_a += ap[1];
     }
     const Cell cell = make_cell(g, st.metallicity[i], st.age[i]);
     double win[4];
     window_sums(g, s.row, s.kl, s.kh, win);
     const double m = s.m, a = s.a, sf = s.sf, T = s.T, Nn = s.Nn;
-    double Pg_m = 0, Pg_u = 0, Pg_s = 0, Pg_a = 0, Pd_u = 0, Pd_s = 0, Pd_a = 0, W_u = 0, W_s = 0;
+    double W_u = 0, W_s = 0;
     for (int c = 0; c < 4; ++c) {
-      const double w = s.w[c], gu = cell.gu[c], gs = cell.gs[c];
-      Pg_m += w * Vg[c];
-      Pg_u += gu * Vg[c];
-      Pg_s += gs * Vg[c];
-      Pg_a += w * Ag[c];
-      Pd_u += gu * Vd[c];
-      Pd_s += gs * Vd[c];
-      Pd_a += w * Ad[c];
-      W_u += gu * win[c];
-      W_s += gs * win[c];
+      W_u += double(cell.gu[c]) * win[c];
+      W_s += double(cell.gs[c]) * win[c];
     }
     const double sbar = m * Pg_m;
     const bool live = isfinite(s.T / s.Nn);
@@ -799,6 +827,7 @@ struct CubeLayout {
   StarState* state;
   ChunkDesc* desc;
   double* partial;
+  float* gradw;    // (n, 8) corner-weight derivatives (gu, gs) per sorted star
   float* nn_part;  // (n, kSlabs) slab partials of Nn
   float* vpart;    // (n, kSlabs, 8) adjoint value sums
   double* apart;   // (n, kSlabs, 8) adjoint Doppler sums
@@ -817,6 +846,7 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, 
   auto* perm = c.template take<int32_t>(n1);
   auto* skey = c.template take<int32_t>(n1);
   auto* state = c.template take<StarState>(n1);
+  auto* gw = c.template take<float>(n1 * 8);
   auto* nnp = c.template take<float>(n1 * kSlabs);
   auto* vp = c.template take<float>(n1 * kSlabs * 8);
   auto* ap = c.template take<double>(n1 * kSlabs * 8);
@@ -825,6 +855,7 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, 
       L->perm = reinterpret_cast<int32_t*>(perm);
       L->skey = reinterpret_cast<int32_t*>(skey);
       L->state = reinterpret_cast<StarState*>(state);
+      L->gradw = reinterpret_cast<float*>(gw);
       L->nn_part = reinterpret_cast<float*>(nnp);
       L->vpart = reinterpret_cast<float*>(vp);
       L->apart = reinterpret_cast<double*>(ap);
@@ -841,6 +872,7 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, 
     L->perm = reinterpret_cast<int32_t*>(perm);
     L->skey = reinterpret_cast<int32_t*>(skey);
     L->state = reinterpret_cast<StarState*>(state);
+    L->gradw = reinterpret_cast<float*>(gw);
     L->nn_part = reinterpret_cast<float*>(nnp);
     L->vpart = reinterpret_cast<float*>(vp);
     L->apart = reinterpret_cast<double*>(ap);
@@ -934,10 +966,10 @@ static void launch_norm(int64_t grid, hipStream_t s, const SspView& g, const rbx
 
 template <int W>
 static void launch_bwd(int64_t grid, hipStream_t s, const SspView& g, const rbx_target& tg,
-                       int64_t n, int64_t items, const StarState* st, const float* dcube,
-                       int slab_bins, float* vp, double* ap) {
+                       int64_t n, int64_t items, const StarState* st, const float* gradw,
+                       const float* dcube, int slab_bins, float* vp, double* ap) {
   hipLaunchKernelGGL(star_bwd_kernel<W>, dim3(grid), dim3(64 * W), 0, s, g, tg, n, items, st,
-                     dcube, slab_bins, vp, ap);
+                     gradw, dcube, slab_bins, vp, ap);
 }
 
 static int validate(const rbx_ssp* ssp, const rbx_target* tg, const rbx_stars* st) {
@@ -1006,7 +1038,7 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   const int tps = (n_tiles + kSlabs - 1) / kSlabs;
   if (n > 0) {
     hipLaunchKernelGGL(star_prep_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *target,
-                       *stars, L.perm, L.skey, L.state);
+                       *stars, L.perm, L.skey, L.state, L.gradw);
     const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
     const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
     switch (t.waves) {
@@ -1062,10 +1094,10 @@ int cube_bwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
   const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
   switch (t.waves) {
-    case 1: launch_bwd<1>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
-    case 2: launch_bwd<2>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
-    case 8: launch_bwd<8>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart); break;
-    default: launch_bwd<4>(grid, s, g, *target, n, items, L.state, dcube, tps * kTile, L.vpart, L.apart);
+    case 1: launch_bwd<1>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart); break;
+    case 2: launch_bwd<2>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart); break;
+    case 8: launch_bwd<8>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart); break;
+    default: launch_bwd<4>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart);
   }
   if ((rc = check_launch(who))) return rc;
   hipLaunchKernelGGL(star_grad_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *stars,
