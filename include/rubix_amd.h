@@ -345,6 +345,49 @@ int rbx_inertia_tensor(const float* coords, const float* mass, int64_t n,
 int rbx_rotate_rows(const float* in, int64_t n, int64_t stride,
                     const rbx_rotation* rot, float* out, void* stream);
 
+/* ---------------------------------------------------------------- noise
+ * The apply_noise step after the LSF (rubix/core/noise.py:17-80 over
+ * rubix/telescope/noise/noise.py:8-115), on a datacube of n_spaxel rows x
+ * n_target bins (row-major, the reference's (n_x, n_y, n_wave) cube):
+ *   flux[p] = sum_l cube[p, l] (double sum, rounded once);
+ *   M       = median of flux if every flux[p] > 0, else 0 (jnp.median of
+ *             where(flux > 0, flux, NaN) propagates the NaN; nan_to_num);
+ *   s2n[p]  = flux[p] > 0 ? (sqrt(M) / signal_to_noise[p]) / sqrt(flux[p]) : 0;
+ *   out     = cube + cube * (z * s2n[p]), each operation rounded in float32.
+ * z[i] at flat index i: jax.random's Threefry-2x32 construction for
+ * PRNGKey(seed) (the reference always uses seed 0): normal (sqrt(2) erfinv of
+ * a uniform on (-1, 1)) or uniform on [0, 1).  The medians' ranks are
+ * selected exactly (radix select); stats gets M and the spaxels at the two
+ * midpoint ranks (stable order), used by the adjoint.  out may alias cube.
+ *
+ * rbx_noise_sample replaces sample_noise (noise.py:8-32): out[i] = z[i].
+ * rbx_apply_noise_bwd: dcube = dout + dout * z * s2n + dL/dflux[p], with
+ * dL/dflux from d s2n / d flux and d M / d flux at the median spaxels
+ * (zero when M = 0); workspace rbx_noise_workspace_size(n_spaxel) bytes.
+ * n_spaxel <= 2^31 - 1; the S2N map is one workgroup's work. */
+#define RBX_NOISE_NORMAL 0
+#define RBX_NOISE_UNIFORM 1
+
+typedef struct rbx_noise_stats {
+  float median;
+  int32_t median_lo;      /* spaxel at rank floor((n-1)/2), -1 if M = 0 */
+  int32_t median_hi;      /* spaxel at rank ceil((n-1)/2), -1 if M = 0 */
+  int32_t any_nonpositive;
+} rbx_noise_stats;
+
+int rbx_noise_sample(int64_t n, uint64_t seed, int32_t distribution, float* out,
+                     void* stream);
+size_t rbx_noise_workspace_size(int32_t n_spaxel);
+int rbx_apply_noise(const float* cube, int32_t n_spaxel, int32_t n_target,
+                    const float* signal_to_noise, uint64_t seed, int32_t distribution,
+                    float* flux, float* s2n, rbx_noise_stats* stats, float* out,
+                    void* stream);
+int rbx_apply_noise_bwd(const float* cube, const float* dout, int32_t n_spaxel,
+                        int32_t n_target, const float* flux, const float* s2n,
+                        const rbx_noise_stats* stats, uint64_t seed,
+                        int32_t distribution, void* workspace, size_t workspace_bytes,
+                        float* dcube, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

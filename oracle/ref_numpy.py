@@ -39,6 +39,20 @@ Pinning (what checks each function, in tests/test_oracle.py):
 * euler_rotation_matrix    - reference known answer for (90, 0, 0)
   (tests/test_galaxy_alignment.py:132-146), scipy Rotation.from_euler
   composition, orthonormal with det 1.
+* threefry2x32             - the published Threefry-2x32-20 known-answer
+  vectors (Random123 kat_vectors; also asserted by jax's random tests).
+* sample_noise             - uniform: the 2^-23 grid, moments, KS test;
+  normal: sqrt(2) erfinv(2p - 1) = ndtri(p) (scipy) for the same p, moments,
+  KS test; the reference's shape / no-NaN / invalid-type cases
+  (tests/test_telescope_noise.py:61-79).  The mapping from key and index to
+  words follows jax's published construction, but jax is absent, so the
+  draws themselves are "parity unpinned" against the reference.
+* flux_median, calculate_S2N - np.median (library) on odd and even counts,
+  NaN propagation, constant-flux closed form S = 1/sn, zero cube.
+* apply_noise              - the reference's cases (zeros -> zero noise, no
+  NaN / inf, variation across spaxels; tests/test_telescope_noise.py:15-58).
+* apply_noise_vjp          - central finite differences of the float64
+  forward (odd and even spaxel counts, so both midpoint weights count).
 """
 from __future__ import annotations
 
@@ -400,6 +414,137 @@ def apply_lsf_kernel(datacube, k, extend_factor=12):
         full[:, t:t + n] += k[t] * spec
     end = n + L - 1 - extend_factor
     return full[:, extend_factor:end].reshape(shape)
+
+
+# ---------------------------------------------------------------- noise
+
+SUPPORTED_NOISE_DISTRIBUTIONS = ["normal", "uniform"]  # rubix/telescope/noise/noise.py:5
+_U32 = np.uint32
+
+
+def _rotl32(x, r):
+    return (x << _U32(r)) | (x >> _U32(32 - r))
+
+
+def threefry2x32(k0, k1, c0, c1):
+    """Threefry-2x32 with 20 rounds (Salmon, Moraes, Dror, Shaw, "Parallel
+    random numbers: as easy as 1, 2, 3", SC 2011), vectorised over the
+    counter words c0, c1 (uint32 arrays): the block function behind
+    jax.random's default PRNG."""
+    rot = (13, 15, 26, 6, 17, 29, 16, 24)
+    ks = (_U32(k0), _U32(k1), _U32(k0) ^ _U32(k1) ^ _U32(0x1BD11BDA))
+    x0 = np.asarray(c0, dtype=_U32) + ks[0]
+    x1 = np.asarray(c1, dtype=_U32) + ks[1]
+    for r in range(20):
+        x0 = x0 + x1
+        x1 = _rotl32(x1, rot[r % 8])
+        x1 = x1 ^ x0
+        if r % 4 == 3:
+            i = r // 4 + 1
+            x0 = x0 + ks[i % 3]
+            x1 = x1 + ks[(i + 1) % 3] + _U32(i)
+    return x0, x1
+
+
+def random_bits(seed, n):
+    """32-bit words for flat indices 0..n-1 under key PRNGKey(seed) =
+    (seed >> 32, seed & 0xffffffff): Threefry of the counter (i >> 32,
+    i & 0xffffffff), the two output words XOR-ed (jax.random's counter layout
+    with jax_threefry_partitionable, the default from jax 0.5)."""
+    i = np.arange(n, dtype=np.uint64)
+    hi = (i >> np.uint64(32)).astype(_U32)
+    lo = (i & np.uint64(0xFFFFFFFF)).astype(_U32)
+    seed = int(seed)
+    y0, y1 = threefry2x32(seed >> 32, seed & 0xFFFFFFFF, hi, lo)
+    return y0 ^ y1
+
+
+def sample_noise(shape, type="normal", seed=0):
+    """rubix/telescope/noise/noise.py:8-32 (key = PRNGKey(seed); the reference
+    passes seed 0).  Uniform: the top 23 bits as the mantissa of a float in
+    [1, 2), minus 1.  Normal: sqrt(2) erfinv(u), u uniform on
+    [nextafter(-1, 0), 1) (jax.random.uniform / normal), float32 throughout
+    with erfinv evaluated in float64 and rounded."""
+    from scipy.special import erfinv
+
+    if type not in SUPPORTED_NOISE_DISTRIBUTIONS:
+        raise ValueError(f"Invalid noise type: {type}. Supported types: "
+                         f"{SUPPORTED_NOISE_DISTRIBUTIONS}")
+    n = int(np.prod(shape))
+    bits = random_bits(seed, n)
+    f = ((bits >> _U32(9)) | _U32(0x3F800000)).view(np.float32) - np.float32(1.0)
+    if type == "uniform":
+        return f.reshape(shape)
+    lo = np.nextafter(np.float32(-1.0), np.float32(0.0))
+    u = np.maximum(lo, f * np.float32(2.0) + lo)
+    e = erfinv(u.astype(np.float64)).astype(np.float32)
+    return (np.float32(np.sqrt(2.0)) * e).reshape(shape)
+
+
+def flux_median(flux):
+    """jnp.median of where(flux > 0, flux, NaN) then nan_to_num
+    (rubix/telescope/noise/noise.py:62-69): NaN (hence 0) as soon as one
+    spaxel has no positive flux, else the midpoint (lo + hi) * 0.5 of the
+    ranks floor((n-1)/2) and ceil((n-1)/2).  Also returns those two spaxels
+    in a stable sort's order (-1, -1 for a zero median)."""
+    f = np.asarray(flux).reshape(-1)
+    if f.size == 0 or not np.all(f > 0):
+        return f.dtype.type(0), -1, -1
+    order = np.argsort(f, kind="stable")
+    n = f.size
+    lo, hi = order[(n - 1) // 2], order[n // 2]
+    return (f[lo] + f[hi]) * f.dtype.type(0.5), int(lo), int(hi)
+
+
+def flux_image(datacube, dtype=np.float32):
+    """rubix/telescope/noise/noise.py:62: sum over the wavelength axis (in
+    float64, rounded once to dtype)."""
+    return np.asarray(datacube, dtype=np.float64).sum(axis=-1).astype(dtype)
+
+
+def calculate_S2N(datacube, signal_to_noise, dtype=np.float32, flux=None):
+    """rubix/telescope/noise/noise.py:35-78 (signal_to_noise a number or the
+    per-spaxel array the apply_noise step passes):
+    S = where(flux > 0, (sqrt(median) / sn) / sqrt(flux), 0), each operation
+    in dtype."""
+    F = flux_image(datacube, dtype) if flux is None else np.asarray(flux, dtype=dtype)
+    med, _, _ = flux_median(F)
+    sn = np.broadcast_to(np.asarray(signal_to_noise, dtype=dtype), F.shape)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        S = (np.sqrt(dtype(med)) / sn) / np.sqrt(F)
+    return np.where(F > 0, S, dtype(0)).astype(dtype)
+
+
+def apply_noise(datacube, signal_to_noise, distribution="normal", seed=0, dtype=np.float32,
+                flux=None):
+    """The apply_noise step (rubix/core/noise.py:67-78 over calculate_noise_cube,
+    rubix/telescope/noise/noise.py:81-115): cube + cube * (z * S[..., None])."""
+    c = np.asarray(datacube, dtype=dtype)
+    z = sample_noise(c.shape, distribution, seed).astype(dtype)
+    S = calculate_S2N(c, signal_to_noise, dtype, flux)
+    return c + c * (z * S[..., None])
+
+
+def apply_noise_vjp(datacube, signal_to_noise, g, distribution="normal", seed=0):
+    """Adjoint of apply_noise with respect to the cube, float64 analytic:
+    d out / d c = 1 + z S (direct) plus the flux path, dS/dF = -S/(2F) and
+    dS/dM = S/(2M) with M moving with the flux of the median spaxels (weight
+    1/2 for each midpoint rank).  M = 0 (a spaxel without positive flux)
+    leaves S = 0 and the adjoint g."""
+    c = np.asarray(datacube, dtype=np.float64)
+    g = np.asarray(g, dtype=np.float64)
+    z = sample_noise(c.shape, distribution, seed).astype(np.float64)
+    F = c.sum(axis=-1)
+    med, lo, hi = flux_median(F)
+    if lo < 0:
+        return g.copy()
+    S = calculate_S2N(c, signal_to_noise, np.float64, F)
+    A = (g * c * z).sum(axis=-1)
+    dM = (A * S).sum() / (2.0 * med)
+    dF = (-A * S / (2.0 * F)).reshape(-1)
+    dF[lo] += 0.5 * dM
+    dF[hi] += 0.5 * dM
+    return g + g * z * S[..., None] + dF.reshape(F.shape)[..., None]
 
 
 # ---------------------------------------------------------------- path

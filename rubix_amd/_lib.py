@@ -7,7 +7,7 @@ runtime per process, so torch's streams and device pointers are valid here.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_float, c_int, c_int32, c_int64, c_size_t, c_void_p
+from ctypes import POINTER, c_float, c_int, c_int32, c_int64, c_size_t, c_uint64, c_void_p
 from pathlib import Path
 
 import torch  # noqa: F401  (load order, see module docstring)
@@ -53,6 +53,11 @@ class RbxRotation(ctypes.Structure):
     _fields_ = [("first", c_float * 9), ("second", c_float * 9)]
 
 
+class RbxNoiseStats(ctypes.Structure):
+    _fields_ = [("median", c_float), ("median_lo", c_int32), ("median_hi", c_int32),
+                ("any_nonpositive", c_int32)]
+
+
 # name -> (restype, argtypes); mirrors include/rubix_amd.h
 SIGNATURES = {
     "rbx_version": (c_int, []),
@@ -88,6 +93,13 @@ SIGNATURES = {
                                    c_void_p, c_size_t, c_void_p]),
     "rbx_rotate_rows": (c_int, [c_void_p, c_int64, c_int64, POINTER(RbxRotation), c_void_p,
                                 c_void_p]),
+    "rbx_noise_sample": (c_int, [c_int64, c_uint64, c_int32, c_void_p, c_void_p]),
+    "rbx_noise_workspace_size": (c_size_t, [c_int32]),
+    "rbx_apply_noise": (c_int, [c_void_p, c_int32, c_int32, c_void_p, c_uint64, c_int32, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rbx_apply_noise_bwd": (c_int, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p,
+                                    c_void_p, c_uint64, c_int32, c_void_p, c_size_t, c_void_p,
+                                    c_void_p]),
     "rbx_ssp_lookup": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "rbx_ssp_lookup_bwd": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),

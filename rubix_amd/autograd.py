@@ -239,6 +239,26 @@ class RotateRows(torch.autograd.Function):
         return ops.rotate_rows(g.contiguous(), second.T, first.T), None, None
 
 
+# ---------------------------------------------------------------- noise
+
+class ApplyNoise(torch.autograd.Function):
+    """apply_noise: cube -> cube + cube * z * S2N(cube).  The adjoint
+    differentiates through the S2N map (flux image and median); the draws z
+    are recomputed from the seed."""
+
+    @staticmethod
+    def forward(ctx, cube, signal_to_noise, distribution, seed):
+        out, state = ops.apply_noise(cube.detach(), signal_to_noise, distribution, seed)
+        ctx.save_for_backward(cube)
+        ctx.state = state
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (cube,) = ctx.saved_tensors
+        return ops.apply_noise_bwd(cube, g.contiguous(), ctx.state), None, None, None
+
+
 # ---------------------------------------------------------------- batches
 
 class BatchedDatacube(torch.autograd.Function):

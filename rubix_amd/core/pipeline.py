@@ -2,10 +2,10 @@
 of the package's pipeline_config.yml and applies them to a RubixData, the
 part of RubixPipeline.run (rubix/core/pipeline.py:93-201) that covers this
 path.  Chains: "calc_ifu" (rotate_galaxy -> filter_particles -> the
-per-step chain -> convolve_lsf, the reference's chain up to its noise step),
+per-step chain -> convolve_lsf -> apply_noise, the reference's chain),
 "calc_ifu_fused" (the same with the fused datacube step), and
-"calc_ifu_aligned" / "calc_ifu_fused_aligned" starting at spaxel_assignment
-for particles that are already aligned and filtered."""
+"calc_ifu_aligned" / "calc_ifu_fused_aligned" from spaxel_assignment to
+convolve_lsf for particles that are already aligned and filtered."""
 from __future__ import annotations
 
 from ..config import pipeline_config
@@ -14,6 +14,7 @@ from .data import RubixData, get_reshape_data
 from .ifu import (get_calculate_datacube, get_calculate_datacube_fused, get_calculate_spectra,
                   get_doppler_shift_and_resampling, get_scale_spectrum_by_mass)
 from .lsf import get_convolve_lsf
+from .noise import get_apply_noise
 from .psf import get_convolve_psf
 from .rotation import get_galaxy_rotation
 from .telescope import get_filter_particles, get_spaxel_assignment
@@ -30,6 +31,7 @@ FACTORIES = {
     "calculate_datacube_fused": get_calculate_datacube_fused,
     "convolve_psf": get_convolve_psf,
     "convolve_lsf": get_convolve_lsf,
+    "apply_noise": get_apply_noise,
 }
 
 

@@ -615,3 +615,95 @@ def segment_cube_bwd(dcube, ids, n_spaxel):
     call("rbx_segment_cube_bwd", ptr(dcube), ptr(ids), n, n_spaxel, dcube.shape[1], ptr(out),
          stream_handle(dcube.device))
     return out
+
+
+# ---------------------------------------------------------------- noise
+
+NOISE_DISTRIBUTIONS = {"normal": 0, "uniform": 1}  # rubix/telescope/noise/noise.py:5
+
+
+def _noise_code(distribution: str) -> int:
+    if distribution not in NOISE_DISTRIBUTIONS:
+        raise ValueError(f"Invalid noise type: {distribution}. Supported types: "
+                         f"{list(NOISE_DISTRIBUTIONS)}")
+    return NOISE_DISTRIBUTIONS[distribution]
+
+
+def _seed(seed: int) -> int:
+    seed = int(seed)
+    if not 0 <= seed < 2 ** 64:
+        raise ValueError(f"seed must be in [0, 2**64), got {seed}")
+    return seed
+
+
+def noise_sample(shape, distribution: str = "normal", seed: int = 0, device=None):
+    """sample_noise (rubix/telescope/noise/noise.py:8-32) for PRNGKey(seed):
+    standard normal or uniform [0, 1) draws at every flat index of shape."""
+    code = _noise_code(distribution)
+    device = torch.device(device if device is not None else "cuda")
+    out = torch.empty(tuple(shape), dtype=torch.float32, device=device)
+    _need_gpu(out)
+    call("rbx_noise_sample", out.numel(), _seed(seed), code, ptr(out), stream_handle(device))
+    return out
+
+
+@dataclass
+class NoiseState:
+    """What the adjoint of apply_noise needs besides the cube: the flux image,
+    the S2N map and the median statistics (device struct)."""
+    flux: torch.Tensor
+    s2n: torch.Tensor
+    stats: torch.Tensor  # 16 bytes: rbx_noise_stats
+    seed: int
+    code: int
+
+    def median(self) -> float:
+        return float(self.stats[:4].view(torch.float32).item())
+
+    def median_spaxels(self):
+        st = self.stats[4:12].view(torch.int32).cpu()
+        return int(st[0]), int(st[1])
+
+
+def _cube_rows(cube):
+    if cube.dim() < 2:
+        raise ValueError(f"datacube must have a wavelength axis, got shape {tuple(cube.shape)}")
+    cube = _f32(cube, "datacube")
+    nt = cube.shape[-1]
+    return cube, cube.numel() // nt if nt else 0, nt
+
+
+def apply_noise(cube, signal_to_noise, distribution: str = "normal", seed: int = 0):
+    """calculate_noise_cube + the datacube update of the apply_noise step
+    (rubix/telescope/noise/noise.py:35-115, rubix/core/noise.py:67-78):
+    returns (cube + noise, NoiseState).  signal_to_noise: a number or a
+    per-spaxel array of cube.shape[:-1]."""
+    _need_gpu(cube)
+    code = _noise_code(distribution)
+    cube, n_spaxel, nt = _cube_rows(cube)
+    sn = torch.as_tensor(signal_to_noise, dtype=torch.float32, device=cube.device)
+    sn = torch.broadcast_to(sn, cube.shape[:-1]).contiguous()
+    dev = cube.device
+    flux = torch.empty(cube.shape[:-1], dtype=torch.float32, device=dev)
+    s2n = torch.empty_like(flux)
+    stats = torch.empty(16, dtype=torch.uint8, device=dev)
+    out = torch.empty_like(cube)
+    call("rbx_apply_noise", ptr(cube), n_spaxel, nt, ptr(sn), _seed(seed), code, ptr(flux),
+         ptr(s2n), ptr(stats), ptr(out), stream_handle(dev))
+    return out, NoiseState(flux, s2n, stats, _seed(seed), code)
+
+
+def apply_noise_bwd(cube, dout, state: NoiseState):
+    """Adjoint of apply_noise with respect to the cube (the draws are
+    recomputed from the seed)."""
+    _need_gpu(cube, dout)
+    cube, n_spaxel, nt = _cube_rows(cube)
+    dout = _f32(dout, "dout")
+    L = _lib.load()
+    wsb = L.rbx_noise_workspace_size(n_spaxel)
+    ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=cube.device)
+    dcube = torch.empty_like(cube)
+    call("rbx_apply_noise_bwd", ptr(cube), ptr(dout), n_spaxel, nt, ptr(state.flux),
+         ptr(state.s2n), ptr(state.stats), state.seed, state.code, ptr(ws), wsb, ptr(dcube),
+         stream_handle(cube.device))
+    return dcube

@@ -218,8 +218,9 @@ SSP_TEMPLATE_NAMES = {(6, 20, 1000): "synthetic_6x20x1000", (12, 100, 5000): "sy
 def step_config(w: Workload, device="cuda"):
     """Step-API config for a workload: a telescope definition with the
     workload's spaxel size and wavelength step, explicit spaxel edges and
-    target wavelengths, the Gaussian PSF (sigma in pixels) and LSF, and the
-    registered synthetic SSP template of the same grid."""
+    target wavelengths, the Gaussian PSF (sigma in pixels) and LSF, noise at
+    S/N 1 (normal), and the registered synthetic SSP template of the same
+    grid."""
     g = w.geom
     nb = g["nb"]
     arcsec = g["fov_arcsec"] / nb
@@ -238,7 +239,8 @@ def step_config(w: Workload, device="cuda"):
         "telescope": {"name": f"SYNTHETIC_{w.name}", "spatial_bin_edges": g["edges"],
                       "wave_seq": g["wave_target"],
                       "psf": {"name": "gaussian", "size": g["psf_size"], "sigma": g["psf_sigma"]},
-                      "lsf": {"sigma": g["lsf_sigma"]}},
+                      "lsf": {"sigma": g["lsf_sigma"]},
+                      "noise": {"signal_to_noise": 1, "noise_distribution": "normal"}},
         "ssp": {"template": {"name": name}, "method": "linear"},
         "cosmology": {"name": "PLANCK15"},
     }

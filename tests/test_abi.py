@@ -35,6 +35,7 @@ def test_struct_layouts_match_static_asserts():
     # batch.hip / conv.hip index rbx_galaxy arrays on the device: 16 bytes each
     assert ctypes.sizeof(_lib.RbxGalaxy) == 16
     assert ctypes.sizeof(_lib.RbxTuning) == 16
+    assert ctypes.sizeof(_lib.RbxNoiseStats) == 16  # noise.hip static_assert
 
 
 def test_version_and_errors():
@@ -72,6 +73,22 @@ def test_batch_arguments_are_validated_before_any_gpu_call():
                   None, None, None, None, None, None, 0, None)
     with pytest.raises(_lib.RbxError, match="rbx_conv_batch"):
         _lib.call("rbx_conv_batch", 8, 16, 24, 1, 4, 10, 2, 32, 3, 1, None)
+
+
+def test_noise_arguments_are_validated_before_any_gpu_call():
+    L = _lib.load()
+    assert L.rbx_noise_workspace_size(2500) == 2500 * 8
+    assert L.rbx_noise_workspace_size(0) == 0
+    with pytest.raises(_lib.RbxError, match="unknown noise distribution 2"):
+        _lib.call("rbx_noise_sample", 10, 0, 2, 8, None)
+    with pytest.raises(_lib.RbxError, match="empty datacube"):
+        _lib.call("rbx_apply_noise", 8, 0, 10, 8, 0, 0, 8, 8, 8, 8, None)
+    with pytest.raises(_lib.RbxError, match="unknown noise distribution"):
+        _lib.call("rbx_apply_noise", 8, 4, 10, 8, 0, 7, 8, 8, 8, 8, None)
+    with pytest.raises(_lib.RbxError, match="null pointer"):
+        _lib.call("rbx_apply_noise", 8, 4, 10, None, 0, 0, 8, 8, 8, 8, None)
+    with pytest.raises(_lib.RbxError, match="workspace of 8 bytes, 32 needed"):
+        _lib.call("rbx_apply_noise_bwd", 8, 8, 4, 10, 8, 8, 8, 0, 1, 8, 8, 8, None)
 
 
 def test_workspace_queries():

@@ -45,8 +45,9 @@ def test_pipeline_matches_oracle(chain):
                                             ("calc_ifu_fused", {"alpha": 45.0, "beta": 30.0,
                                                                 "gamma": 15.0})])
 def test_full_chain_from_raw_particles(chain, rotation):
-    """rotate_galaxy -> filter_particles -> ... -> convolve_lsf against the
-    oracle's alignment, aperture filter and plain forward (float64)."""
+    """rotate_galaxy -> filter_particles -> ... -> convolve_lsf -> apply_noise
+    against the oracle's alignment, aperture filter, plain forward and noise
+    (float64)."""
     w = S.config1(n_stars=4000, seed=7)
     cfg = S.step_config(w)
     cfg["galaxy"]["rotation"] = rotation
@@ -64,6 +65,7 @@ def test_full_chain_from_raw_particles(chain, rotation):
                                       {k: st[k] for k in ("mass", "age", "metallicity")})
     ref_stars = dict(coords=p.astype(np.float32), velocity=v.astype(np.float32), **fields)
     ref = R.plain_forward(ref_stars, w.grid, w.geom, dtype=np.float64)
+    ref = R.apply_noise(ref, 1.0, "normal", 0, dtype=np.float64)
     got_p = out.stars.coords.detach().cpu().numpy().reshape(-1, 3)
     assert np.abs(got_p - p).max() <= 1e-5 * np.abs(p).max()
     np.testing.assert_array_equal(out.stars.mask.cpu().numpy().reshape(-1), mask)
