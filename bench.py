@@ -119,9 +119,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("single", "large", "batched"), default="single",
+    ap.add_argument("--workload", choices=("single", "large", "batched", "noise"),
+                    default="single",
                     help="single: 1M stars -> 50x50x3721 (the headline); large: 10M stars -> "
-                         "150x150x3721; batched: 64 ragged galaxies in one pass")
+                         "150x150x3721; batched: 64 ragged galaxies in one pass; noise: the "
+                         "apply_noise step on a 50x50x3721 cube")
     ap.add_argument("--n-stars", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
@@ -145,6 +147,8 @@ def main():
         return run_large(args)
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "noise":
+        return run_noise(args)
     return run_single(args)
 
 
@@ -490,6 +494,79 @@ def run_batched(args):
         "loop_plain_per_galaxy_ms": None if plain_loop_ms is None else round(plain_loop_ms, 3),
         "speedup_vs_plain_loop": None if plain_loop_ms is None else round(plain_loop_ms / ms, 3),
         "speedup_vs_fused_loop": round(fused_loop_ms / ms, 3),
+    }
+    print(json.dumps(res), flush=True)
+
+
+
+def run_noise(args):
+    """apply_noise forward + adjoint on a 50 x 50 x 3721 cube with positive
+    flux in every spaxel (so the S2N map is non-zero everywhere and every
+    element is drawn), normal noise at S/N 1, seed 0."""
+    import numpy as np
+    import torch
+
+    from rubix_amd import _lib
+    from rubix_amd.autograd import ApplyNoise
+
+    dev = torch.device("cuda", 0)
+    nb, nt = 50, 3721
+    rng = np.random.default_rng(0)
+    c_np = rng.uniform(0.5, 2.0, (nb, nb, nt)).astype(np.float32)
+    g = torch.as_tensor(rng.normal(size=(nb, nb, nt)).astype(np.float32), device=dev)
+    x = torch.as_tensor(c_np, device=dev).requires_grad_(True)
+
+    def step(ev):
+        if ev:
+            ev[0].record()
+        out = ApplyNoise.apply(x, 1.0, "normal", 0)
+        if ev:
+            ev[1].record()
+        out.backward(g)
+        if ev:
+            ev[2].record()
+        x.grad = None
+
+    _lib.TIMER.start()
+    elapsed, fwd_ms, adj_ms = timed_leg(step, args.warmup, args.steps)
+    calls = _lib.TIMER.stop()
+    per_call = {k: float(np.median(v[-args.steps:])) for k, v in calls.items()}
+    el = nb * nb * nt
+    ms = elapsed / args.steps * 1e3
+    # forward: cube read by the flux pass and the noise pass, out written (12 B);
+    # adjoint: dout and cube read, dcube written, then dcube += dL/dF (20 B)
+    fwd_b, adj_b = 12 * el, 20 * el
+    dom = max(("rbx_apply_noise", fwd_b), ("rbx_apply_noise_bwd", adj_b),
+              key=lambda kv: per_call.get(kv[0], 0.0))
+    k_ms = per_call.get(dom[0], float("nan"))
+    achieved = dom[1] / (k_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom[0], "kernel_ms": round(k_ms, 4),
+                "algorithmic_bytes_per_launch": dom[1]}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ref_numpy as R
+
+        t0 = time.perf_counter()
+        R.apply_noise(c_np, 1.0, "normal", 0)
+        R.apply_noise_vjp(c_np, 1.0, g.cpu().numpy(), "normal", 0)
+        dt = time.perf_counter() - t0
+        cpu = {"value": el / dt, "unit": "cube elements/s", "cores": 1, "kind": "port",
+               "sample": f"the whole {nb}x{nb}x{nt} cube: numpy restatement (float32 forward, "
+                         f"float64 adjoint), one pass {dt:.1f}s on {_model_name()}"}
+    res = {
+        "metric": "cube elements/s, apply_noise forward + adjoint", "value": el / (ms * 1e-3),
+        "unit": "cube elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"apply_noise on a {nb}x{nb}x{nt} cube (all spaxels with positive "
+                               "flux), normal, S/N 1, seed 0; forward + adjoint",
+                   "n_spaxel": nb * nb, "n_target": nt},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "split": {"fwd_ms_median": round(float(np.median(fwd_ms)), 4),
+                  "adj_ms_median": round(float(np.median(adj_ms)), 4)},
+        "per_call_ms": {k: round(v, 4) for k, v in per_call.items()},
     }
     print(json.dumps(res), flush=True)
 

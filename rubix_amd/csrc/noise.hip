@@ -23,6 +23,7 @@ static_assert(sizeof(rbx_noise_stats) == 16, "rbx_noise_stats layout");
 namespace rbx {
 
 constexpr int kSelThreads = 1024;  // the one workgroup of the median / S2N kernel
+constexpr int kSelStage = 8192;    // flux images up to this many spaxels sit in LDS
 constexpr int kRowThreads = 256;
 constexpr int kRowSeg = 1024;  // bins per elementwise item
 
@@ -89,31 +90,52 @@ __device__ inline double block_sum(double v, double* sm) {
   return tot;
 }
 
-// Per spaxel row p: flux F[p] = sum_l cube (kDot false), or for the adjoint
-// A[p] = sum_l dout * cube * z (kDot true; rows with S[p] == 0 need no A and
-// get 0).  Double accumulation, fixed order.
-template <bool kDot>
-__global__ void __launch_bounds__(kRowThreads) row_sum_kernel(
-    const float* __restrict__ cube, const float* __restrict__ dout, int n_spaxel, int nt,
-    const float* __restrict__ s2n, NoiseKey key, int uniform, float* __restrict__ flux,
-    double* __restrict__ dots) {
+// Flux image: F[p] = sum_l cube[p, l], one workgroup per spaxel row,
+// double accumulation in a fixed order, rounded once.
+__global__ void __launch_bounds__(kRowThreads) flux_rows_kernel(const float* __restrict__ cube,
+                                                                int n_spaxel, int nt,
+                                                                float* __restrict__ flux) {
   __shared__ double sm[kRowThreads / kWave];
   for (int p = blockIdx.x; p < n_spaxel; p += gridDim.x) {
     const int64_t row = int64_t(p) * nt;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int l = threadIdx.x;
+    for (; l + 3 * kRowThreads < nt; l += 4 * kRowThreads) {
+      a0 += cube[row + l];
+      a1 += cube[row + l + kRowThreads];
+      a2 += cube[row + l + 2 * kRowThreads];
+      a3 += cube[row + l + 3 * kRowThreads];
+    }
+    for (; l < nt; l += kRowThreads) a0 += cube[row + l];
+    const double tot = block_sum((a0 + a1) + (a2 + a3), sm);
+    if (threadIdx.x == 0) flux[p] = float(tot);
+  }
+}
+
+// Adjoint, first pass over spaxel row p (each draw made once): the direct
+// term dcube = dout + dout * (z * S[p]) and A[p] = sum_l dout * cube * z
+// (double, fixed order).  Rows with S[p] == 0 copy dout and get A = 0.
+__global__ void __launch_bounds__(kRowThreads) noise_bwd_rows_kernel(
+    const float* __restrict__ cube, const float* __restrict__ dout, int n_spaxel, int nt,
+    const float* __restrict__ s2n, NoiseKey key, int uniform, double* __restrict__ dots,
+    float* __restrict__ dcube) {
+  __shared__ double sm[kRowThreads / kWave];
+  for (int p = blockIdx.x; p < n_spaxel; p += gridDim.x) {
+    const int64_t row = int64_t(p) * nt;
+    const float s = s2n[p];
     double acc = 0.0;
-    if (!kDot) {
-      for (int l = threadIdx.x; l < nt; l += kRowThreads) acc += cube[row + l];
-    } else if (s2n[p] != 0.f) {
+    if (s != 0.f) {
       for (int l = threadIdx.x; l < nt; l += kRowThreads) {
         const float z = draw(key, uint64_t(row + l), uniform);
-        acc += double(dout[row + l]) * double(cube[row + l]) * double(z);
+        const float g = dout[row + l];
+        acc += double(g) * double(cube[row + l]) * double(z);
+        dcube[row + l] = g + g * (z * s);
       }
+    } else {
+      for (int l = threadIdx.x; l < nt; l += kRowThreads) dcube[row + l] = dout[row + l];
     }
     const double tot = block_sum(acc, sm);
-    if (threadIdx.x == 0) {
-      if (kDot) dots[p] = tot;
-      else flux[p] = float(tot);
-    }
+    if (threadIdx.x == 0) dots[p] = tot;
   }
 }
 
@@ -121,8 +143,8 @@ __global__ void __launch_bounds__(kRowThreads) row_sum_kernel(
 // patterns order like the values: 4 radix passes of 8 bits with an LDS
 // histogram.  Returns the value's bits; *tie gets k's rank among the
 // elements equal to it.
-__device__ uint32_t radix_select(const float* __restrict__ F, int n, int k, unsigned* hist,
-                                 uint32_t* s_pref, int* s_k, int* tie) {
+__device__ uint32_t radix_select(const float* F, int n, int k, unsigned* hist, uint32_t* s_pref,
+                                 int* s_k, int* tie) {
   uint32_t prefix = 0, mask = 0;
   for (int shift = 24; shift >= 0; shift -= 8) {
     for (int d = threadIdx.x; d < 256; d += blockDim.x) hist[d] = 0;
@@ -132,15 +154,32 @@ __device__ uint32_t radix_select(const float* __restrict__ F, int n, int k, unsi
       if ((b & mask) == prefix) atomicAdd(&hist[(b >> shift) & 255u], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned cum = 0;
-      int d = 0;
-      for (; d < 255; ++d) {
-        if (cum + hist[d] > unsigned(k)) break;
-        cum += hist[d];
+    if (threadIdx.x < kWave) {
+      // wave 0: lane q owns bins 4q..4q+3; an inclusive shuffle scan finds
+      // the one lane whose bins hold rank k (counts sum to more than k)
+      const int lane = threadIdx.x;
+      unsigned h[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) h[j] = hist[4 * lane + j];
+      const unsigned own = h[0] + h[1] + h[2] + h[3];
+      unsigned incl = own;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned t = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += t;
       }
-      *s_pref = prefix | (uint32_t(d) << shift);
-      *s_k = k - int(cum);
+      unsigned cum = incl - own;
+      if (cum <= unsigned(k) && unsigned(k) < incl) {
+        int d = 4 * lane;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          if (cum + h[j] > unsigned(k)) break;
+          cum += h[j];
+          ++d;
+        }
+        *s_pref = prefix | (uint32_t(d) << shift);
+        *s_k = k - int(cum);
+      }
     }
     __syncthreads();
     prefix = *s_pref;
@@ -154,7 +193,7 @@ __device__ uint32_t radix_select(const float* __restrict__ F, int n, int k, unsi
 
 // Index of the rank-th (0-based) element whose bits equal `bits`, in index
 // order (the element a stable sort puts at that rank).
-__device__ int nth_equal(const float* __restrict__ F, int n, uint32_t bits, int rank,
+__device__ int nth_equal(const float* F, int n, uint32_t bits, int rank,
                          unsigned* wave_cnt, int* s_idx) {
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int n_waves = blockDim.x / kWave;
@@ -187,15 +226,22 @@ __device__ int nth_equal(const float* __restrict__ F, int n, uint32_t bits, int 
 __global__ void __launch_bounds__(kSelThreads) noise_s2n_kernel(
     const float* __restrict__ F, int n, const float* __restrict__ sn, float* __restrict__ s2n,
     rbx_noise_stats* __restrict__ stats) {
+  __shared__ float stage[kSelStage];
   __shared__ unsigned hist[256];
   __shared__ unsigned wave_cnt[kSelThreads / kWave];
   __shared__ uint32_t s_pref;
   __shared__ int s_k, s_idx, s_any;
   if (threadIdx.x == 0) s_any = 0;
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x)
-    if (!(F[i] > 0.f)) s_any = 1;
+  // the selection passes read the flux image from LDS when it fits
+  const bool staged = n <= kSelStage;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float f = F[i];
+    if (!(f > 0.f)) s_any = 1;
+    if (staged) stage[i] = f;
+  }
   __syncthreads();
+  const float* src = staged ? stage : F;
   const bool valid = s_any == 0;
   float med = 0.f;
   int lo_idx = -1, hi_idx = -1;
@@ -203,13 +249,13 @@ __global__ void __launch_bounds__(kSelThreads) noise_s2n_kernel(
     // midpoint of ranks floor((n-1)/2) and ceil((n-1)/2): (lo + hi) * 0.5
     const int k_lo = (n - 1) / 2, k_hi = n / 2;
     int tie;
-    const uint32_t b_lo = radix_select(F, n, k_lo, hist, &s_pref, &s_k, &tie);
-    lo_idx = nth_equal(F, n, b_lo, tie, wave_cnt, &s_idx);
+    const uint32_t b_lo = radix_select(src, n, k_lo, hist, &s_pref, &s_k, &tie);
+    lo_idx = nth_equal(src, n, b_lo, tie, wave_cnt, &s_idx);
     uint32_t b_hi = b_lo;
     hi_idx = lo_idx;
     if (k_hi != k_lo) {
-      b_hi = radix_select(F, n, k_hi, hist, &s_pref, &s_k, &tie);
-      hi_idx = nth_equal(F, n, b_hi, tie, wave_cnt, &s_idx);
+      b_hi = radix_select(src, n, k_hi, hist, &s_pref, &s_k, &tie);
+      hi_idx = nth_equal(src, n, b_hi, tie, wave_cnt, &s_idx);
     }
     med = (__uint_as_float(b_lo) + __uint_as_float(b_hi)) * 0.5f;
   }
@@ -288,25 +334,18 @@ __global__ void __launch_bounds__(kSelThreads) noise_coef_kernel(
   }
 }
 
-// dcube = dout + dout * (z * S[p]) + dL/dF[p]
-__global__ void __launch_bounds__(kRowThreads) add_noise_bwd_kernel(
-    const float* __restrict__ dout, int nt, uint32_t n_items, uint32_t segs,
-    const float* __restrict__ s2n, const double* __restrict__ dF, NoiseKey key, int uniform,
-    float* __restrict__ dcube) {
+// Adjoint, last pass: dcube[p, l] += dL/dF[p] (the flux path).
+__global__ void __launch_bounds__(kRowThreads) add_rows_kernel(float* __restrict__ dcube, int nt,
+                                                               uint32_t n_items, uint32_t segs,
+                                                               const double* __restrict__ dF) {
   for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const RowItem it = row_item(item, segs);
-    const float s = s2n[it.p];
     const float b = float(dF[it.p]);
     const int64_t row = int64_t(it.p) * nt;
 #pragma unroll
     for (int q = 0; q < kRowSeg / kRowThreads; ++q) {
       const int l = it.l0 + q * kRowThreads + threadIdx.x;
-      if (l < nt) {
-        const float g = dout[row + l];
-        float d = g;
-        if (s != 0.f) d = g + g * (draw(key, uint64_t(row + l), uniform) * s);
-        dcube[row + l] = d + b;
-      }
+      if (l < nt) dcube[row + l] += b;
     }
   }
 }
@@ -360,8 +399,8 @@ int rbx_apply_noise(const float* cube, int32_t n_spaxel, int32_t n_target,
   hipStream_t st = as_stream(stream);
   const NoiseKey key = key_of(seed);
   const int uniform = distribution == RBX_NOISE_UNIFORM ? 1 : 0;
-  hipLaunchKernelGGL(row_sum_kernel<false>, dim3(row_grid(n_spaxel)), dim3(kRowThreads), 0, st,
-                     cube, nullptr, n_spaxel, n_target, nullptr, key, uniform, flux, nullptr);
+  hipLaunchKernelGGL(flux_rows_kernel, dim3(row_grid(n_spaxel)), dim3(kRowThreads), 0, st, cube,
+                     n_spaxel, n_target, flux);
   hipLaunchKernelGGL(noise_s2n_kernel, dim3(1), dim3(kSelThreads), 0, st, flux, n_spaxel,
                      signal_to_noise, s2n, stats);
   const uint32_t segs = uint32_t((n_target + kRowSeg - 1) / kRowSeg);
@@ -387,14 +426,14 @@ int rbx_apply_noise_bwd(const float* cube, const float* dout, int32_t n_spaxel, 
   const NoiseKey key = key_of(seed);
   const int uniform = distribution == RBX_NOISE_UNIFORM ? 1 : 0;
   double* dots = static_cast<double*>(workspace);
-  hipLaunchKernelGGL(row_sum_kernel<true>, dim3(row_grid(n_spaxel)), dim3(kRowThreads), 0, st,
-                     cube, dout, n_spaxel, n_target, s2n, key, uniform, nullptr, dots);
+  hipLaunchKernelGGL(noise_bwd_rows_kernel, dim3(row_grid(n_spaxel)), dim3(kRowThreads), 0, st,
+                     cube, dout, n_spaxel, n_target, s2n, key, uniform, dots, dcube);
   hipLaunchKernelGGL(noise_coef_kernel, dim3(1), dim3(kSelThreads), 0, st, n_spaxel, flux, s2n,
                      stats, dots);
   const uint32_t segs = uint32_t((n_target + kRowSeg - 1) / kRowSeg);
   const uint32_t n_items = uint32_t(n_spaxel) * segs;
-  hipLaunchKernelGGL(add_noise_bwd_kernel, dim3(item_blocks(n_items)), dim3(kRowThreads), 0, st,
-                     dout, n_target, n_items, segs, s2n, dots, key, uniform, dcube);
+  hipLaunchKernelGGL(add_rows_kernel, dim3(item_blocks(n_items)), dim3(kRowThreads), 0, st, dcube,
+                     n_target, n_items, segs, dots);
   return check_launch("rbx_apply_noise_bwd");
 }
 
