@@ -388,6 +388,57 @@ int rbx_apply_noise_bwd(const float* cube, const float* dout, int32_t n_spaxel,
                         int32_t distribution, void* workspace, size_t workspace_bytes,
                         float* dcube, void* stream);
 
+/* ---------------------------------------------------------------- dust
+ * The calculate_extinction step of calc_dusty_ifu (rubix/core/dust.py:16-65
+ * over rubix/spectra/dust/dust_extinction.py:163-358).
+ *
+ * rbx_extinction_curve: axav[l] = A(x)/A(V) of Cardelli89
+ * (extinction_models.py:104-178) or Gordon23 (:255-445) at x = wave[l] / 1e4,
+ * the argument the reference's step passes (microns; its Cardelli89 branches
+ * treat the argument as a wavenumber).
+ *
+ * rbx_gas_extinction: per gas cell (metals rows of metals_stride >= 5
+ * columns, H in column 0, O in column 4): log(O/H) + 12 = 12 +
+ * log10(O / (16 H)), the Remy-Ruyer et al. (2014) dust-to-gas ratio
+ * (dust_extinction.py:13-90), dust mass = mass * D/G and
+ * ext = 3 pi dust C / (0.4 ln 10 * 5448e-8 * grain_density) / spaxel_area with
+ * C = MSUN_TO_GRAMS / KPC_TO_CM^2 (:93-160, 269-287).
+ *
+ * rbx_column_av: A_V per star (dust_extinction.py:237-327): for the star's
+ * spaxel s (stars with ids outside [0, n_spaxels) get 0), jnp.interp with
+ * left = "extrapolate" at the star's z over every gas cell, keyed z for
+ * cells of s and fl(z * 1e30) for the others, valued the inclusive
+ * cumulative extinction of s's cells in z order (0 for the others), in the
+ * stable order of a lexsort by (pixel, z).  Coordinates are rows of
+ * *_stride >= 3 floats (z in column 2); gas pixel ids come from
+ * rbx_spaxel_assign, sorted over n_keys buckets (ids >= n_keys after them).
+ * If any gas extinction is not finite, every A_V is NaN.
+ * Workspace: rbx_column_av_workspace_size(n_gas, n_keys) bytes.
+ *
+ * rbx_apply_extinction: out[i, l] = in[i, l] * 10^((-0.4 axav[l]) * av[i])
+ * over (n, n_wave) rows; the same call is its own adjoint; out may alias
+ * in. */
+#define RBX_EXT_CARDELLI89 0
+#define RBX_EXT_GORDON23 1
+#define RBX_DTG_POWER_LAW_FREE 0  /* "power law slope free" */
+#define RBX_DTG_BROKEN_POWER_LAW 1 /* "broken power law fit" */
+#define RBX_XCO_MW 0
+#define RBX_XCO_Z 1
+
+int rbx_extinction_curve(int32_t model, float rv, const float* wave, int32_t n,
+                         float* axav, void* stream);
+int rbx_gas_extinction(const float* mass, const float* metals, int64_t n,
+                       int64_t metals_stride, int32_t dtg_model, int32_t xco,
+                       float grain_density, float spaxel_area, float* ext, void* stream);
+size_t rbx_column_av_workspace_size(int64_t n_gas, int32_t n_keys);
+int rbx_column_av(const float* gas_coords, int64_t gas_stride, const int32_t* gas_pix,
+                  const float* gas_ext, int64_t n_gas, const float* star_coords,
+                  int64_t star_stride, const int32_t* star_pix, int64_t n_star,
+                  int32_t n_spaxels, int32_t n_keys, void* workspace,
+                  size_t workspace_bytes, float* av, void* stream);
+int rbx_apply_extinction(const float* in, const float* av, const float* axav, int64_t n,
+                         int32_t n_wave, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

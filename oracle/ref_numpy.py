@@ -268,11 +268,12 @@ def calculate_diff(vec, pad_with_zero=True):
     return np.diff(vec, axis=-1)
 
 
-def interp_jax(x, xp, fp):
+def interp_jax(x, xp, fp, left=None):
     """``jnp.interp`` semantics (used at rubix/spectra/ifu.py:250):
     i = clip(searchsorted(xp, x, 'right'), 1, n-1);
     f = fp[i-1] + (x - xp[i-1]) / (xp[i] - xp[i-1]) * (fp[i] - fp[i-1]);
-    f = fp[0] where x < xp[0]; f = fp[-1] where x > xp[-1]."""
+    f = fp[0] where x < xp[0] (unless left == "extrapolate");
+    f = fp[-1] where x > xp[-1]."""
     x = np.asarray(x)
     xp = np.asarray(xp)
     fp = np.asarray(fp)
@@ -285,7 +286,8 @@ def interp_jax(x, xp, fp):
     dx0 = np.abs(dx) <= eps
     with np.errstate(divide="ignore", invalid="ignore"):
         f = np.where(dx0, fp[i - 1], fp[i - 1] + (delta / np.where(dx0, 1, dx)) * df)
-    f = np.where(x < xp[0], fp[0], f)
+    if left != "extrapolate":
+        f = np.where(x < xp[0], fp[0], f)
     f = np.where(x > xp[-1], fp[-1], f)
     return f.astype(fp.dtype)
 
@@ -545,6 +547,243 @@ def apply_noise_vjp(datacube, signal_to_noise, g, distribution="normal", seed=0)
     dF[lo] += 0.5 * dM
     dF[hi] += 0.5 * dM
     return g + g * z * S[..., None] + dF.reshape(F.shape)[..., None]
+
+
+# ---------------------------------------------------------------- dust
+
+# rubix/config/rubix_config.yml:3-7, 145-150
+MSUN_TO_GRAMS = 1.989e33
+KPC_TO_CM = 3.08568e21
+RV_MODELS = ["Cardelli89", "Gordon23"]  # rubix/spectra/dust/extinction_models.py:10
+
+
+def powerlaw1d(x, amplitude, x_0, alpha):
+    """rubix/spectra/dust/generic_models.py:18-48: A (x / x0)^-alpha."""
+    return amplitude * (x / x_0) ** (-alpha)
+
+
+def polynomial1d(x, coeffs):
+    """generic_models.py:51-97 with the default domain = window = (-1, 1)
+    (the identity map): Horner from the highest coefficient."""
+    coeffs = list(coeffs)
+    if len(coeffs) == 1:
+        return coeffs[-1] * np.ones_like(x)
+    c0 = coeffs[-1]
+    for i in range(2, len(coeffs) + 1):
+        c0 = coeffs[-i] + c0 * x
+    return c0
+
+
+def drude1d(x, amplitude=1.0, x_0=1.0, fwhm=1.0):
+    """generic_models.py:100-155."""
+    if x_0 == 0:
+        raise ValueError("0 is not an allowed value for x_0")
+    return amplitude * ((fwhm / x_0) ** 2) / ((x / x_0 - x_0 / x) ** 2 + (fwhm / x_0) ** 2)
+
+
+def modified_drude(x, scale, x_o, gamma_o, asym):
+    """generic_models.py:158-194."""
+    gamma = 2.0 * gamma_o / (1.0 + np.exp(asym * (x - x_o)))
+    return scale * ((gamma / x_o) ** 2) / ((x / x_o - x_o / x) ** 2 + (gamma / x_o) ** 2)
+
+
+def fm90(x, C1=0.10, C2=0.70, C3=3.23, C4=0.41, xo=4.59, gamma=0.95):
+    """generic_models.py:197-335 (Fitzpatrick & Massa 1990), with the
+    reference's parameter bounds and messages."""
+    bounds = {"C1": (-10.0, 5.0), "C2": (-0.1, 5.0), "C3": (-1.0, 6.0), "C4": (-0.5, 1.5),
+              "xo": (4.5, 4.9), "gamma": (0.6, 1.7)}
+    for k, v in (("C1", C1), ("C2", C2), ("C3", C3), ("C4", C4), ("xo", xo), ("gamma", gamma)):
+        if not (bounds[k][0] <= v <= bounds[k][1]):
+            raise ValueError(f"{k} is out of bounds: {v}")
+    e = C1 + C2 * x
+    x2 = x ** 2
+    e = e + C3 * (x2 / ((x2 - xo ** 2) ** 2 + x2 * (gamma ** 2)))
+    m = x >= 5.9
+    y = np.where(m, x - 5.9, 0.0)
+    return np.where(m, e + C4 * (0.5392 * (y ** 2) + 0.05644 * (y ** 3)), e)
+
+
+def smoothstep1(x, x_min, x_max):
+    """helpers.py:68-84 with N = 1: t = clip((x - x_min)/(x_max - x_min), 0, 1),
+    (comb(1,0) comb(3,1) - comb(2,1) comb(3,0) t) t^2 = (3 - 2t) t^2."""
+    t = np.clip((x - x_min) / (x_max - x_min), 0, 1)
+    return (3.0 - 2.0 * t) * t ** 2
+
+
+def cardelli89(wave, Rv):
+    """A(x)/A(V) of rubix/spectra/dust/extinction_models.py:104-178 (CCM89).
+    The branches test the argument as a wavenumber x in 1/micron; the
+    reference's dust step passes wavelength / 1e4, i.e. microns (see
+    apply_spaxel_extinction)."""
+    w = np.asarray(wave, dtype=np.float64)
+    a = np.zeros_like(w)
+    b = np.zeros_like(w)
+    ir = (0.3 <= w) & (w < 1.1)
+    opt = (1.1 <= w) & (w < 3.3)
+    nuv = (3.3 <= w) & (w <= 8.0)
+    fnuv = (5.9 <= w) & (w <= 8)
+    fuv = (8 < w) & (w <= 10)
+    a = np.where(ir, 0.574 * w ** 1.61, a)
+    b = np.where(ir, -0.527 * w ** 1.61, b)
+    y = w - 1.82
+    a = np.where(opt, 1 + 0.17699 * y - 0.50447 * y ** 2 - 0.02427 * y ** 3 + 0.72085 * y ** 4
+                 + 0.01979 * y ** 5 - 0.77530 * y ** 6 + 0.32999 * y ** 7, a)
+    b = np.where(opt, 1.41338 * y + 2.28305 * y ** 2 + 1.07233 * y ** 3 - 5.38434 * y ** 4
+                 - 0.62251 * y ** 5 + 5.30260 * y ** 6 - 2.09002 * y ** 7, b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a = np.where(nuv, 1.752 - 0.316 * w - 0.104 / ((w - 4.67) ** 2 + 0.341), a)
+        b = np.where(nuv, -3.09 + 1.825 * w + 1.206 / ((w - 4.62) ** 2 + 0.263), b)
+    y = w - 5.9
+    a = np.where(fnuv, a + (-0.04473 * y ** 2 - 0.009779 * y ** 3), a)
+    b = np.where(fnuv, b + (0.2130 * y ** 2 + 0.1207 * y ** 3), b)
+    y = w - 8.0
+    a = np.where(fuv, -1.073 - 0.628 * y + 0.137 * y ** 2 - 0.070 * y ** 3, a)
+    b = np.where(fuv, 13.670 + 4.257 * y - 0.420 * y ** 2 + 0.374 * y ** 3, b)
+    return a + b / Rv
+
+
+_G23_IR_A = [0.38526, 1.68467, 0.78791, 4.30578, 4.78338,
+             0.06652, 9.8434, 2.21205, -0.24703, 0.0267, 19.58294, 17., -0.27]
+_G23_OPT_A = [-0.35848, 0.7122, 0.08746, -0.05403, 0.00674,
+              0.03893, 2.288, 0.243, 0.02965, 2.054, 0.179, 0.01747, 1.587, 0.243]
+_G23_OPT_B = [0.12354, -2.68335, 2.01901, -0.39299, 0.03355,
+              0.18453, 2.288, 0.243, 0.19728, 2.054, 0.179, 0.1713, 1.587, 0.243]
+
+
+def _g23_nirmir(w, p):
+    """extinction_models.py:401-445: broken power law + two modified Drudes."""
+    scale, alpha, alpha2, swave, swidth = p[:5]
+    pow1 = scale * w ** (-1.0 * alpha)
+    ratio = swave ** (-1.0 * alpha) / swave ** (-1.0 * alpha2)
+    pow2 = scale * ratio * w ** (-1.0 * alpha2)
+    wt = smoothstep1(w, swave - swidth / 2, swave + swidth / 2)
+    ax = pow1 * (1.0 - wt) + pow2 * wt
+    ax = ax + modified_drude(w, *p[5:9])
+    return ax + modified_drude(w, *p[9:13])
+
+
+def _g23_optical(x, p):
+    """extinction_models.py:315-351: polynomial (5 coefficients) + 3 Drudes."""
+    return (polynomial1d(x, p[:5]) + drude1d(x, p[5], p[6], p[7]) + drude1d(x, p[8], p[9], p[10])
+            + drude1d(x, p[11], p[12], p[13]))
+
+
+def gordon23(wave, Rv):
+    """A(lambda)/A(V) of extinction_models.py:255-398 (G23), wave in microns."""
+    w = np.asarray(wave, dtype=np.float64)
+    a = np.zeros_like(w)
+    b = np.zeros_like(w)
+    ir = (1.0 <= w) & (w < 35.0)
+    opt = (0.3 <= w) & (w < 1.1)
+    uv = (0.09 <= w) & (w <= 0.3)
+    optir = (w >= 0.9) & (w <= 1.1)
+    uvopt = (w >= 0.3) & (w <= 0.33)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        ir_a = _g23_nirmir(w, _G23_IR_A)
+        ir_b = powerlaw1d(w, -1.01251, 1.0, -1.06099)
+        op_a = _g23_optical(1 / w, _G23_OPT_A)
+        op_b = _g23_optical(1 / w, _G23_OPT_B)
+        uv_a = fm90(1 / w, 0.81297, 0.2775, 1.06295, 0.11303, 4.60, 0.99)
+        uv_b = fm90(1 / w, -2.97868, 1.89808, 3.10334, 0.65484, 4.60, 0.99)
+    a = np.where(ir, ir_a, a)
+    b = np.where(ir, ir_b, b)
+    a = np.where(opt, op_a, a)
+    b = np.where(opt, op_b, b)
+    wt = smoothstep1(w, 0.9, 1.1)
+    a = np.where(optir, (1.0 - wt) * op_a + wt * ir_a, a)
+    b = np.where(optir, (1.0 - wt) * op_b + wt * ir_b, b)
+    a = np.where(uv, uv_a, a)
+    b = np.where(uv, uv_b, b)
+    wt = smoothstep1(w, 0.3, 0.33)
+    a = np.where(uvopt, (1.0 - wt) * uv_a + wt * op_a, a)
+    b = np.where(uvopt, (1.0 - wt) * uv_b + wt * op_b, b)
+    return a + b * (1 / Rv - 1 / 3.1)
+
+
+def extinction_curve(model, wave_um, Rv):
+    if model not in RV_MODELS:
+        raise ValueError(f"Extinction model '{model}' is not available. Choose from {RV_MODELS}.")
+    return cardelli89(wave_um, Rv) if model == "Cardelli89" else gordon23(wave_um, Rv)
+
+
+def extinguish(axav, Av):
+    """dust_baseclasses.py:125-164: 10^(-0.4 A(x)/A(V) Av)."""
+    return np.power(10.0, -0.4 * axav * Av)
+
+
+def dust_to_gas_ratio(log_OH, model, Xco):
+    """rubix/spectra/dust/dust_extinction.py:13-90 (Remy-Ruyer et al. 2014,
+    table 1)."""
+    x_sol = 8.69
+    if model == "power law slope fixed":
+        raise NotImplementedError("power law slope fixed not implemented yet.")
+    table = {("MW", "power law slope free"): (2.21, 1.62),
+             ("Z", "power law slope free"): (2.21, 2.02)}
+    if (Xco, model) in table:
+        a, alpha = table[(Xco, model)]
+        return 1 / 10 ** (a + alpha * (x_sol - log_OH))
+    broken = {"MW": (2.21, 1.00, 0.68, 3.08, 7.96), "Z": (2.21, 1.00, 0.96, 3.10, 8.10)}
+    if model == "broken power law fit" and Xco in broken:
+        a, ah, b, al, xt = broken[Xco]
+        return 1 / np.where(log_OH > xt, 10 ** (a + ah * (x_sol - log_OH)),
+                            10 ** (b + al * (x_sol - log_OH)))
+    raise ValueError(f"unknown dust-to-gas model {model!r} / Xco {Xco!r}")
+
+
+def gas_extinction(gas_mass, gas_metals, model, Xco, grain_density, spaxel_area):
+    """Per gas cell: log(O/H) + 12 from metals[:, 4] / (16 metals[:, 0]),
+    dust mass = mass * D/G, and calculate_extinction
+    (dust_extinction.py:93-160, 3 pi Sigma C / (0.4 ln 10 lambda_V rho)) over
+    the spaxel area (dust_extinction.py:269-287)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        log_OH = 12 + np.log10(gas_metals[:, 4] / (16 * gas_metals[:, 0]))
+        dg = dust_to_gas_ratio(log_OH, model, Xco)
+        dust = gas_mass * dg
+        conv = MSUN_TO_GRAMS / KPC_TO_CM ** 2
+        ext = 3 * np.pi * dust * conv / (0.4 * np.log(10) * (5448 * 1e-8) * grain_density)
+    return ext / spaxel_area
+
+
+def column_av(gas_z, gas_pix, gas_ext, star_z, star_pix, n_spaxels):
+    """Av per star, the scan of dust_extinction.py:237-340 restated literally:
+    gas in lexsort (pixel, z) order; for each spaxel s the keys are
+    z * where(own, 1, 1e30) (float32), the values the masked cumulative sum
+    of the sorted extinction, stably sorted by key, and jnp.interp with
+    left = "extrapolate" at each star's z; a star keeps the value of its
+    own spaxel.  gas_ext is per gas cell in input order.  float32."""
+    f = np.float32
+    gz = np.asarray(gas_z, f)
+    order = np.lexsort((gz, np.asarray(gas_pix)))
+    gz_s = gz[order]
+    gp_s = np.asarray(gas_pix)[order]
+    ext_s = np.asarray(gas_ext, f)[order]
+    sz = np.asarray(star_z, f)
+    sp = np.asarray(star_pix)
+    av = np.zeros(len(sz), f)
+    big = f(1e30)
+    for s in range(n_spaxels):
+        own = gp_s == s
+        with np.errstate(invalid="ignore", over="ignore"):
+            cum = np.cumsum(ext_s * own.astype(f), dtype=f) * own.astype(f)
+            keys = gz_s * np.where(own, f(1), big)
+        o = np.argsort(keys, kind="stable")
+        xp, fp = keys[o], cum[o]
+        stars = sp == s
+        if not stars.any() or len(xp) == 0:
+            continue
+        if len(xp) == 1:
+            av[stars] = fp[0]
+            continue
+        with np.errstate(invalid="ignore", divide="ignore"):
+            av[stars] = interp_jax(sz[stars], xp, fp, left="extrapolate")
+    return av
+
+
+def apply_spaxel_extinction(spectra, wave_target, av, model, Rv):
+    """spectra (N, n_wave) times 10^(-0.4 A(lambda)/A(V) Av) with the curve
+    evaluated at wave_target / 1e4 (dust_extinction.py:330-358)."""
+    ax = extinction_curve(model, np.asarray(wave_target, np.float64) / 1e4, Rv)
+    return np.asarray(spectra) * extinguish(ax[None, :], np.asarray(av)[:, None])
 
 
 # ---------------------------------------------------------------- path

@@ -100,6 +100,15 @@ SIGNATURES = {
     "rbx_apply_noise_bwd": (c_int, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p,
                                     c_void_p, c_uint64, c_int32, c_void_p, c_size_t, c_void_p,
                                     c_void_p]),
+    "rbx_extinction_curve": (c_int, [c_int32, c_float, c_void_p, c_int32, c_void_p, c_void_p]),
+    "rbx_gas_extinction": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_int32, c_int32, c_float,
+                                   c_float, c_void_p, c_void_p]),
+    "rbx_column_av_workspace_size": (c_size_t, [c_int64, c_int32]),
+    "rbx_column_av": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int64,
+                              c_void_p, c_int64, c_int32, c_int32, c_void_p, c_size_t, c_void_p,
+                              c_void_p]),
+    "rbx_apply_extinction": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_void_p,
+                                     c_void_p]),
     "rbx_ssp_lookup": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "rbx_ssp_lookup_bwd": (c_int, [POINTER(RbxSsp), c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),

@@ -259,6 +259,25 @@ class ApplyNoise(torch.autograd.Function):
         return ops.apply_noise_bwd(cube, g.contiguous(), ctx.state), None, None, None
 
 
+# ---------------------------------------------------------------- dust
+
+class ApplyExtinction(torch.autograd.Function):
+    """Per-star spectra (N, n_wave) times 10^(-0.4 A(lambda)/A(V) A_V); the
+    adjoint multiplies by the same factor.  A_V and the curve are constants
+    (they depend on coordinates and gas, not on the differentiated star
+    fields)."""
+
+    @staticmethod
+    def forward(ctx, spectra, av, axav):
+        ctx.save_for_backward(av, axav)
+        return ops.apply_extinction(spectra.detach(), av, axav)
+
+    @staticmethod
+    def backward(ctx, g):
+        av, axav = ctx.saved_tensors
+        return ops.apply_extinction(g.contiguous(), av, axav), None, None
+
+
 # ---------------------------------------------------------------- batches
 
 class BatchedDatacube(torch.autograd.Function):

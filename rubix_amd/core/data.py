@@ -129,14 +129,21 @@ def get_reshape_data(config: dict) -> Callable:
 
 
 def stars_from_arrays(coords, velocity, mass, age, metallicity, device="cuda",
-                      requires_grad=False, halfmassrad_stars=None) -> RubixData:
+                      requires_grad=False, halfmassrad_stars=None, gas=None) -> RubixData:
     """Convenience: RubixData with the star fields as float32 device tensors
-    (and the galaxy's stellar half-mass radius, which rotate_galaxy reads)."""
+    (and the galaxy's stellar half-mass radius, which rotate_galaxy reads).
+    gas: optional dict of gas arrays (coords, velocity, mass, metals, ...)."""
     def t(x):
         v = torch.as_tensor(np.asarray(x, dtype=np.float32), device=device)
         return v.requires_grad_(requires_grad)
 
+    def plain(x):
+        return torch.as_tensor(np.asarray(x, np.float32), device=device)
+
+    gas_data = None
+    if gas is not None:
+        gas_data = GasData(**{k: plain(v) for k, v in gas.items()})
     return RubixData(galaxy=Galaxy(halfmassrad_stars=halfmassrad_stars),
-                     stars=StarsData(coords=torch.as_tensor(np.asarray(coords, np.float32), device=device),
-                                     velocity=t(velocity), mass=t(mass), age=t(age),
-                                     metallicity=t(metallicity)))
+                     stars=StarsData(coords=plain(coords), velocity=t(velocity), mass=t(mass),
+                                     age=t(age), metallicity=t(metallicity)),
+                     gas=gas_data)

@@ -3,7 +3,8 @@ of the package's pipeline_config.yml and applies them to a RubixData, the
 part of RubixPipeline.run (rubix/core/pipeline.py:93-201) that covers this
 path.  Chains: "calc_ifu" (rotate_galaxy -> filter_particles -> the
 per-step chain -> convolve_lsf -> apply_noise, the reference's chain),
-"calc_ifu_fused" (the same with the fused datacube step), and
+"calc_ifu_fused" (the same with the fused datacube step), "calc_dusty_ifu"
+(calc_ifu with calculate_extinction after the resampling), and
 "calc_ifu_aligned" / "calc_ifu_fused_aligned" from spaxel_assignment to
 convolve_lsf for particles that are already aligned and filtered."""
 from __future__ import annotations
@@ -13,6 +14,7 @@ from ..pipeline import LinearTransformerPipeline
 from .data import RubixData, get_reshape_data
 from .ifu import (get_calculate_datacube, get_calculate_datacube_fused, get_calculate_spectra,
                   get_doppler_shift_and_resampling, get_scale_spectrum_by_mass)
+from .dust import get_extinction
 from .lsf import get_convolve_lsf
 from .noise import get_apply_noise
 from .psf import get_convolve_psf
@@ -27,6 +29,7 @@ FACTORIES = {
     "calculate_spectra": get_calculate_spectra,
     "scale_spectrum_by_mass": get_scale_spectrum_by_mass,
     "doppler_shift_and_resampling": get_doppler_shift_and_resampling,
+    "calculate_extinction": get_extinction,
     "calculate_datacube": get_calculate_datacube,
     "calculate_datacube_fused": get_calculate_datacube_fused,
     "convolve_psf": get_convolve_psf,

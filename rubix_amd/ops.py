@@ -707,3 +707,82 @@ def apply_noise_bwd(cube, dout, state: NoiseState):
          ptr(state.s2n), ptr(state.stats), state.seed, state.code, ptr(ws), wsb, ptr(dcube),
          stream_handle(cube.device))
     return dcube
+
+
+# ---------------------------------------------------------------- dust
+
+RV_MODELS = {"Cardelli89": 0, "Gordon23": 1}  # rubix/spectra/dust/extinction_models.py:10
+DUST_TO_GAS_MODELS = {"power law slope free": 0, "broken power law fit": 1}
+XCO_MODELS = {"MW": 0, "Z": 1}
+
+
+def extinction_curve(model: str, rv: float, wave):
+    """A(x)/A(V) of the Cardelli89 or Gordon23 model at wave / 1e4 (wave in
+    Angstrom, as rubix/spectra/dust/dust_extinction.py:351 passes it)."""
+    if model not in RV_MODELS:
+        raise ValueError(f"Extinction model '{model}' is not available. Choose from "
+                         f"{list(RV_MODELS)}.")
+    _need_gpu(wave)
+    wave = _f32(wave, "wave").reshape(-1)
+    out = torch.empty_like(wave)
+    call("rbx_extinction_curve", RV_MODELS[model], float(rv), ptr(wave), wave.numel(), ptr(out),
+         stream_handle(wave.device))
+    return out
+
+
+def gas_extinction(mass, metals, dust_to_gas_model: str, xco: str, grain_density: float,
+                   spaxel_area: float):
+    """Per gas cell V-band extinction over the spaxel area
+    (rubix/spectra/dust/dust_extinction.py:13-160, 269-287)."""
+    if dust_to_gas_model == "power law slope fixed":
+        raise NotImplementedError("power law slope fixed not implemented yet.")
+    if dust_to_gas_model not in DUST_TO_GAS_MODELS:
+        raise ValueError(f"unknown dust-to-gas model {dust_to_gas_model!r}; supported: "
+                         f"{list(DUST_TO_GAS_MODELS)}")
+    if xco not in XCO_MODELS:
+        raise ValueError(f"unknown Xco {xco!r}; supported: {list(XCO_MODELS)}")
+    _need_gpu(mass, metals)
+    mass = _f32(mass, "gas mass").reshape(-1)
+    metals = _f32(metals, "gas metals")
+    metals = metals.reshape(mass.numel(), -1)
+    if metals.shape[1] < 5:
+        raise ValueError(f"gas metals need at least 5 columns (H ... O), got {metals.shape[1]}")
+    out = torch.empty_like(mass)
+    call("rbx_gas_extinction", ptr(mass), ptr(metals), mass.numel(), metals.shape[1],
+         DUST_TO_GAS_MODELS[dust_to_gas_model], XCO_MODELS[xco], float(grain_density),
+         float(spaxel_area), ptr(out), stream_handle(mass.device))
+    return out
+
+
+def column_av(gas_coords, gas_pix, gas_ext, star_coords, star_pix, n_spaxels: int, n_keys: int):
+    """A_V per star: the cumulative gas extinction of its spaxel along z at
+    the star's z (rubix/spectra/dust/dust_extinction.py:237-327)."""
+    _need_gpu(gas_coords, gas_pix, gas_ext, star_coords, star_pix)
+    gc = _rows(gas_coords.reshape(-1, gas_coords.shape[-1]), "gas coords")
+    sc = _rows(star_coords.reshape(-1, star_coords.shape[-1]), "star coords")
+    gp = gas_pix.reshape(-1).to(torch.int32).contiguous()
+    sp = star_pix.reshape(-1).to(torch.int32).contiguous()
+    ge = _f32(gas_ext, "gas extinction").reshape(-1)
+    L = _lib.load()
+    wsb = L.rbx_column_av_workspace_size(gc.shape[0], n_keys)
+    ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=sc.device)
+    av = torch.empty(sc.shape[0], dtype=torch.float32, device=sc.device)
+    call("rbx_column_av", ptr(gc), gc.shape[1], ptr(gp), ptr(ge), gc.shape[0], ptr(sc), sc.shape[1],
+         ptr(sp), sc.shape[0], int(n_spaxels), int(n_keys), ptr(ws), wsb, ptr(av),
+         stream_handle(sc.device))
+    return av
+
+
+def apply_extinction(spectra, av, axav):
+    """spectra (N, n_wave) * 10^(-0.4 axav * av) (dust_extinction.py:330-358)."""
+    _need_gpu(spectra, av, axav)
+    x = _rows(spectra, "spectra")
+    av = _f32(av, "av").reshape(-1)
+    axav = _f32(axav, "axav").reshape(-1)
+    if av.numel() != x.shape[0] or axav.numel() != x.shape[1]:
+        raise ValueError(f"apply_extinction: spectra {tuple(x.shape)}, av {av.numel()}, "
+                         f"axav {axav.numel()}")
+    out = torch.empty_like(x)
+    call("rbx_apply_extinction", ptr(x), ptr(av), ptr(axav), x.shape[0], x.shape[1], ptr(out),
+         stream_handle(x.device))
+    return out

@@ -85,6 +85,29 @@ def make_stars(n, seed=0, h_r=3.0, h_z=0.3, v_rot=200.0, sigma_v=50.0,
                 age=age.astype(f32), metallicity=met.astype(f32))
 
 
+def make_gas(n, seed=0, h_r=3.0, h_z=0.2, mass_median=2e4, mass_sigma_ln=0.7,
+             oxygen_range=(2e-3, 1.5e-2)):
+    """Synthetic gas cells: an exponential-ish disk like make_stars, masses in
+    solar masses, and metals (n, 10) mass fractions with hydrogen in column 0
+    and oxygen in column 4 (the columns the dust step reads)."""
+    rng = np.random.default_rng(seed)
+    R = rng.gamma(2.0, h_r, n)
+    phi = rng.uniform(0.0, 2 * math.pi, n)
+    zh = rng.laplace(0.0, h_z, n)
+    pos = np.stack([R * np.cos(phi), R * np.sin(phi), zh], axis=1)
+    vel = rng.normal(0.0, 30.0, (n, 3))
+    mass = np.exp(rng.normal(math.log(mass_median), mass_sigma_ln, n))
+    metals = np.zeros((n, 10))
+    metals[:, 0] = rng.uniform(0.70, 0.76, n)
+    metals[:, 1] = 0.25
+    metals[:, 4] = 10.0 ** rng.uniform(math.log10(oxygen_range[0]), math.log10(oxygen_range[1]), n)
+    metals[:, 2:4] = 1e-3
+    metals[:, 5:] = 5e-4
+    f32 = np.float32
+    return dict(coords=pos.astype(f32), velocity=vel.astype(f32), mass=mass.astype(f32),
+                metals=metals.astype(f32))
+
+
 def _planck(lam_A, T):
     lam = lam_A * 1e-10
     hc_k = 1.438776877e-2  # m K
@@ -241,6 +264,7 @@ def step_config(w: Workload, device="cuda"):
                       "psf": {"name": "gaussian", "size": g["psf_size"], "sigma": g["psf_sigma"]},
                       "lsf": {"sigma": g["lsf_sigma"]},
                       "noise": {"signal_to_noise": 1, "noise_distribution": "normal"}},
-        "ssp": {"template": {"name": name}, "method": "linear"},
+        "ssp": {"template": {"name": name}, "method": "linear",
+                "dust": {"extinction_model": "Cardelli89", "Rv": 3.1, "dust_grain_density": 3.5}},
         "cosmology": {"name": "PLANCK15"},
     }

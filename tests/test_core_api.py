@@ -125,4 +125,4 @@ def test_linear_pipeline_order_and_errors():
 
 def test_pipeline_unknown_name():
     with pytest.raises(ValueError, match="not found"):
-        RubixPipeline({"pipeline": {"name": "calc_dusty_ifu"}})
+        RubixPipeline({"pipeline": {"name": "no_such_chain"}})
