@@ -159,6 +159,27 @@ int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target,
                  float* dage, float* dmet, float* dvlos, void* workspace,
                  size_t workspace_bytes, void* stream);
 
+/* Dust extinction inside the fused path (the calc_dusty_ifu chain of
+ * rubix/config/pipeline_config.yml:61-131 without materialised spectra):
+ * each (star, bin) value is multiplied by 10^((-0.4 axav[j]) * av[star]),
+ * the factor rbx_apply_extinction applies to resampled spectra
+ * (rubix/spectra/dust/dust_extinction.py:330-358).  av (n, input order) from
+ * rbx_column_av, axav (target.n) from rbx_extinction_curve.  The adjoint
+ * takes the same descriptor (it reads av from the forward's workspace state;
+ * A_V is a constant of the differentiation).  NULL = no extinction. */
+typedef struct rbx_extinction {
+  const float* av;
+  const float* axav;
+} rbx_extinction;
+
+int rbx_cube_fwd_ext(const rbx_ssp* ssp, const rbx_target* target,
+                     const rbx_stars* stars, const rbx_extinction* ext, float* cube,
+                     void* workspace, size_t workspace_bytes, void* stream);
+int rbx_cube_bwd_ext(const rbx_ssp* ssp, const rbx_target* target,
+                     const rbx_stars* stars, const rbx_extinction* ext,
+                     const float* dcube, float* dmass, float* dage, float* dmet,
+                     float* dvlos, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------- batches
  * Many galaxies with uneven star counts and spaxel grids in one pass (the
  * reference renders one galaxy per pipeline run, rubix/core/pipeline.py;

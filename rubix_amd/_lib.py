@@ -53,6 +53,10 @@ class RbxRotation(ctypes.Structure):
     _fields_ = [("first", c_float * 9), ("second", c_float * 9)]
 
 
+class RbxExtinction(ctypes.Structure):
+    _fields_ = [("av", c_void_p), ("axav", c_void_p)]
+
+
 class RbxNoiseStats(ctypes.Structure):
     _fields_ = [("median", c_float), ("median_lo", c_int32), ("median_hi", c_int32),
                 ("any_nonpositive", c_int32)]
@@ -72,6 +76,11 @@ SIGNATURES = {
                              c_void_p, c_size_t, c_void_p]),
     "rbx_cube_bwd": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars), c_void_p,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "rbx_cube_fwd_ext": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars),
+                                 POINTER(RbxExtinction), c_void_p, c_void_p, c_size_t, c_void_p]),
+    "rbx_cube_bwd_ext": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars),
+                                 POINTER(RbxExtinction), c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_size_t, c_void_p]),
     "rbx_batch_workspace_size": (c_size_t, [c_int64, c_int32, c_int32, POINTER(RbxTuning)]),
     "rbx_batch_fwd": (c_int, [POINTER(RbxSsp), POINTER(RbxTarget), POINTER(RbxStars), c_void_p,
                               c_int32, POINTER(RbxTuning), c_void_p, c_void_p, c_size_t,

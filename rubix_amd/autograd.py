@@ -106,9 +106,11 @@ class FusedDatacube(torch.autograd.Function):
     """(mass, age, metallicity, velocity) -> raw datacube (nb*nb, n_target)."""
 
     @staticmethod
-    def forward(ctx, mass, age, met, velocity, ids, ssp, target, n_spaxel, n_keys, vel_comp, c):
+    def forward(ctx, mass, age, met, velocity, ids, ssp, target, n_spaxel, n_keys, vel_comp, c,
+                extinction=None):
         cube, cctx = ops.cube_forward(ssp, target, mass.detach(), age.detach(), met.detach(),
-                                      velocity.detach(), ids, n_spaxel, n_keys, vel_comp, c)
+                                      velocity.detach(), ids, n_spaxel, n_keys, vel_comp, c,
+                                      extinction)
         ctx.cctx = cctx
         ctx.ssp = ssp
         ctx.target = target
@@ -125,7 +127,7 @@ class FusedDatacube(torch.autograd.Function):
         if need[3]:
             dvel = torch.zeros(ctx.vshape, dtype=torch.float32, device=dcube.device)
             dvel[:, ctx.vel_comp] = dv
-        return dm, dt, dz, dvel, None, None, None, None, None, None, None
+        return dm, dt, dz, dvel, None, None, None, None, None, None, None, None
 
 
 # ---------------------------------------------------------------- PSF

@@ -39,8 +39,9 @@ def spaxel_area(config: dict) -> float:
     return float(np.float32(size) * np.float32(size))
 
 
-def get_extinction(config: dict) -> Callable:
-    logger = get_logger(config.get("logger", None))
+def extinction_inputs(config: dict) -> Callable:
+    """Validated dust config -> f(rubixdata) = (A_V per star in input order,
+    A(lambda)/A(V) per telescope wavelength), both device tensors."""
     if "dust" not in config["ssp"]:
         raise ValueError("Dust configuration not found in config file.")
     if "extinction_model" not in config["ssp"]["dust"]:
@@ -53,8 +54,7 @@ def get_extinction(config: dict) -> Callable:
     wave = torch.as_tensor(np.asarray(tel.wave_seq, dtype=np.float32), device=dev)
     area = spaxel_area(config)
 
-    def calculate_extinction(rubixdata: RubixData) -> RubixData:
-        logger.info("Applying dust extinction to the spaxel data...")
+    def inputs(rubixdata: RubixData):
         model = dust["extinction_model"]
         if model not in RV_MODELS:
             raise ValueError(
@@ -75,9 +75,39 @@ def get_extinction(config: dict) -> Callable:
         n_keys = n_spaxels if edges is None else max(n_spaxels, (int(edges.numel()) - 1) ** 2)
         av = ops.column_av(gc, gas.pixel_assignment.reshape(-1), ext, sc,
                            stars.pixel_assignment.reshape(-1), n_spaxels, n_keys)
-        s = stars.spectra
+        return av, axav
+
+    return inputs
+
+
+def get_extinction(config: dict) -> Callable:
+    logger = get_logger(config.get("logger", None))
+    inputs = extinction_inputs(config)
+
+    def calculate_extinction(rubixdata: RubixData) -> RubixData:
+        logger.info("Applying dust extinction to the spaxel data...")
+        av, axav = inputs(rubixdata)
+        s = rubixdata.stars.spectra
         out = ApplyExtinction.apply(s.reshape(-1, s.shape[-1]), av, axav)
-        stars.spectra = out.reshape(s.shape)
+        rubixdata.stars.spectra = out.reshape(s.shape)
         return rubixdata
 
     return calculate_extinction
+
+
+def get_calculate_dusty_datacube_fused(config: dict) -> Callable:
+    """calculate_spectra -> scale_spectrum_by_mass -> doppler_shift_and_resampling
+    -> calculate_extinction -> calculate_datacube in one fused pass: the
+    datacube step of the fused calc_dusty_ifu chain.  Writes stars.datacube
+    (nb, nb, n_target); per-star spectra are never materialised."""
+    from .ifu import get_calculate_datacube_fused
+
+    logger = get_logger(config.get("logger", None))
+    inputs = extinction_inputs(config)
+    fused = get_calculate_datacube_fused(config, with_extinction=True)
+
+    def calculate_dusty_datacube_fused(rubixdata: RubixData) -> RubixData:
+        logger.info("Calculating Data Cube with dust extinction (fused)...")
+        return fused(rubixdata, inputs(rubixdata))
+
+    return calculate_dusty_datacube_fused

@@ -127,10 +127,13 @@ def get_calculate_datacube(config: dict) -> Callable:
     return calculate_datacube
 
 
-def get_calculate_datacube_fused(config: dict) -> Callable:
+def get_calculate_datacube_fused(config: dict, with_extinction: bool = False) -> Callable:
     """Fused replacement of calculate_spectra -> scale_spectrum_by_mass ->
     doppler_shift_and_resampling -> calculate_datacube: writes
-    stars.datacube (nb, nb, n_target), leaves stars.spectra None."""
+    stars.datacube (nb, nb, n_target), leaves stars.spectra None.  With
+    with_extinction the returned step takes (rubixdata, (av, axav)) and
+    attenuates every star's contribution (the calculate_extinction step,
+    rubix/spectra/dust/dust_extinction.py:330-358)."""
     logger = get_logger(config.get("logger", None))
     lookup_method(config)
     comp = _los_component()
@@ -143,8 +146,9 @@ def get_calculate_datacube_fused(config: dict) -> Callable:
     target = ops.TargetGrid(tel.wave_seq, dev)
     c = _speed_of_light()
 
-    def calculate_datacube_fused(rubixdata: RubixData) -> RubixData:
-        logger.info("Calculating Data Cube (fused)...")
+    def calculate_datacube_fused(rubixdata: RubixData, extinction=None) -> RubixData:
+        if not with_extinction:
+            logger.info("Calculating Data Cube (fused)...")
         st = rubixdata.stars
         mass = _dev(st.mass, dev).reshape(-1)
         age = _dev(st.age, dev).reshape(-1)
@@ -153,7 +157,7 @@ def get_calculate_datacube_fused(config: dict) -> Callable:
         vel = vel.reshape(-1, vel.shape[-1])
         ids = st.pixel_assignment.to(dev).reshape(-1).to(torch.int32)
         cube = FusedDatacube.apply(mass, age, met, vel, ids, prepared, target, nb * nb, nb * nb,
-                                   comp, c)
+                                   comp, c, extinction)
         st.datacube = cube.reshape(nb, nb, target.n)
         st.spectra = None
         return rubixdata

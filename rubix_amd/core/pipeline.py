@@ -4,7 +4,8 @@ part of RubixPipeline.run (rubix/core/pipeline.py:93-201) that covers this
 path.  Chains: "calc_ifu" (rotate_galaxy -> filter_particles -> the
 per-step chain -> convolve_lsf -> apply_noise, the reference's chain),
 "calc_ifu_fused" (the same with the fused datacube step), "calc_dusty_ifu"
-(calc_ifu with calculate_extinction after the resampling), and
+(calc_ifu with calculate_extinction after the resampling), "calc_dusty_ifu_fused"
+(the same with one fused dusty datacube step), and
 "calc_ifu_aligned" / "calc_ifu_fused_aligned" from spaxel_assignment to
 convolve_lsf for particles that are already aligned and filtered."""
 from __future__ import annotations
@@ -14,7 +15,7 @@ from ..pipeline import LinearTransformerPipeline
 from .data import RubixData, get_reshape_data
 from .ifu import (get_calculate_datacube, get_calculate_datacube_fused, get_calculate_spectra,
                   get_doppler_shift_and_resampling, get_scale_spectrum_by_mass)
-from .dust import get_extinction
+from .dust import get_calculate_dusty_datacube_fused, get_extinction
 from .lsf import get_convolve_lsf
 from .noise import get_apply_noise
 from .psf import get_convolve_psf
@@ -32,6 +33,7 @@ FACTORIES = {
     "calculate_extinction": get_extinction,
     "calculate_datacube": get_calculate_datacube,
     "calculate_datacube_fused": get_calculate_datacube_fused,
+    "calculate_dusty_datacube_fused": get_calculate_dusty_datacube_fused,
     "convolve_psf": get_convolve_psf,
     "convolve_lsf": get_convolve_lsf,
     "apply_noise": get_apply_noise,

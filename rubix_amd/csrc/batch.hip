@@ -86,7 +86,7 @@ int rbx_batch_fwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
                        galaxies, n_gal, stars->n_spaxel, keys);
   }
   const rbx_stars k = keyed(stars, keys);
-  return cube_fwd_impl(ssp, target, &k, t, cube, static_cast<char*>(workspace) + kb,
+  return cube_fwd_impl(ssp, target, &k, nullptr, t, cube, static_cast<char*>(workspace) + kb,
                        workspace_bytes - kb, s, "rbx_batch_fwd");
 }
 
@@ -103,7 +103,7 @@ int rbx_batch_bwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   if (!workspace || workspace_bytes < kb)
     return fail(RBX_EWORKSPACE, "rbx_batch_bwd: workspace too small");
   const rbx_stars k = keyed(stars, static_cast<int32_t*>(workspace));
-  return cube_bwd_impl(ssp, target, &k, t, dcube, dmass, dage, dmet, dvlos,
+  return cube_bwd_impl(ssp, target, &k, nullptr, t, dcube, dmass, dage, dmet, dvlos,
                        static_cast<char*>(workspace) + kb, workspace_bytes - kb,
                        as_stream(stream), "rbx_batch_bwd");
 }

@@ -53,7 +53,7 @@ struct __align__(16) StarState {
   float m, T, sf, Nn;
   int32_t kl, kh;   // T window: knots kl..kh (kl >= 1)
   int32_t spaxel;   // segment id (raw id from the assignment)
-  int32_t pad;
+  float av;         // dust A_V of the star (0 without extinction)
 };
 static_assert(sizeof(StarState) == 64, "StarState must be 64 bytes");
 // The C-ABI descriptors are mirrored by ctypes structures; pin their layout.
@@ -77,7 +77,7 @@ __device__ inline void window_sums(const SspView& g, int row, int kl, int kh, do
 
 __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
                                  const int32_t* __restrict__ perm,
-                                 const int32_t* __restrict__ skey,
+                                 const int32_t* __restrict__ skey, const float* __restrict__ av,
                                  StarState* __restrict__ out, float* __restrict__ gradw) {
   const int64_t n = st.n;
   for (int64_t r = gtid(); r < n; r += int64_t(gridDim.x) * blockDim.x) {
@@ -86,7 +86,7 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
     StarState s;
     s.orig = i;
     s.spaxel = key;
-    s.pad = 0;
+    s.av = av ? av[i] : 0.f;
     const float m = st.mass[i];
     const float v = st.velocity[int64_t(i) * st.vel_stride + st.vel_comp];
     const float a = doppler_factor(v, st.speed_of_light);
@@ -459,13 +459,25 @@ __global__ void chunk_fill_kernel(const int32_t* __restrict__ offsets, int32_t n
 
 // ------------------------------------------------------------ accumulate
 
+// Dust attenuation of bin j for a star with extinction A_V:
+// 10^((-0.4 A(lambda_j)/A(V)) * A_V), the factor the calculate_extinction
+// step multiplies into the resampled spectra (each product rounded on its own).
+__device__ inline float dust_tau(const float* axav, uint32_t j) {
+  return mul_nocontract(-0.4f, ld_at(axav, j));
+}
+__device__ inline float dust_factor(float tau, float av) {
+  return exp10f(mul_nocontract(tau, av));
+}
+
 // Item q: 64-bin lambda tile of XCD q % 8's slab and one spaxel chunk; the
 // items are strided over the grid like the per-star kernels' (G a multiple
 // of 8), so workgroup b stays on slab b % 8.
+template <bool kDust>
 __global__ void __launch_bounds__(64) cube_accum_kernel(
     SspView g, rbx_target tg, const StarState* __restrict__ state,
     const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr,
-    int n_tiles, int tiles_per_xcd, int64_t n_items, double* __restrict__ partial) {
+    int n_tiles, int tiles_per_xcd, int64_t n_items, const float* __restrict__ axav,
+    double* __restrict__ partial) {
   const int n_chunks = *n_chunks_ptr;
   for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
     const int xcd = int(q & 7);
@@ -477,6 +489,7 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
     const int j = tile * kTile + int(threadIdx.x);
     const bool valid = j < tg.n;
     const float x = ld_at(tg.wave, uint32_t(valid ? j : tg.n - 1));
+    const float tau = kDust ? dust_tau(axav, uint32_t(valid ? j : tg.n - 1)) : 0.f;
     // double accumulator: the cube is a sum of up to chunk-size positive
     // terms per lane; accumulating in double keeps the forward's rounding
     // at the level of the individual float32 terms.
@@ -487,7 +500,9 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
       if (s.row < 0) continue;
       const Rows R = rows_of(g, s.row);
       const Knots K = knots_of<false>(g, x, shift_of(g, s.a, s.inv_a));
-      acc += double(star_value(R, s, K) * s.sf);
+      float v = star_value(R, s, K) * s.sf;
+      if (kDust && s.av != 0.f) v *= dust_factor(tau, s.av);
+      acc += double(v);
     }
     if (valid) partial[int64_t(chunk) * tg.n + j] = acc;
   }
@@ -503,11 +518,11 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
 // bitwise those of the per-tile kernel.
 constexpr int kLaneBins = 8;
 
-template <int W>
+template <int W, bool kDust>
 __global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
     SspView g, rbx_target tg, const StarState* __restrict__ state,
     const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr, int slab_bins,
-    int64_t n_items, double* __restrict__ partial) {
+    int64_t n_items, const float* __restrict__ axav, double* __restrict__ partial) {
   __shared__ float lam_s[W][kWindow];
   __shared__ float F_s[W][kWindow];
   const int n_chunks = *n_chunks_ptr;
@@ -523,12 +538,13 @@ __global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
     const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
     if (j0 >= j1) continue;
     const ChunkDesc d = desc[chunk];
-    float x[kLaneBins];
+    float x[kLaneBins], tau[kLaneBins];
     double acc[kLaneBins];
 #pragma unroll
     for (int t = 0; t < kLaneBins; ++t) {
       const uint32_t j = j0 + lane + 64u * t;
       x[t] = ld_at(tg.wave, j < j1 ? j : j1 - 1);
+      tau[t] = kDust ? dust_tau(axav, j < j1 ? j : j1 - 1) : 0.f;
       acc[t] = 0.0;
     }
     for (int r = d.begin; r < d.end; ++r) {
@@ -553,7 +569,9 @@ __global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
           if (j0 + lane + 64u * t < j1) {
             const WinKnots K = win_knots<false>(g, x[t], h, k_lo, nk, lam);
             const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
-            acc[t] += double((S0 + K.theta * (S1 - S0)) * s.sf);
+            float v = (S0 + K.theta * (S1 - S0)) * s.sf;
+            if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
+            acc[t] += double(v);
           }
         }
         wave_lds_sync();
@@ -562,7 +580,9 @@ __global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
         for (int t = 0; t < kLaneBins; ++t) {
           if (j0 + lane + 64u * t < j1) {
             const Knots K = knots_of<false>(g, x[t], h);
-            acc[t] += double(star_value(R, s, K) * s.sf);
+            float v = star_value(R, s, K) * s.sf;
+            if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
+            acc[t] += double(v);
           }
         }
       }
@@ -632,11 +652,11 @@ __device__ inline double lds_sum(const float v[K], float* buf, int lane) {
 // corners per knot takes differences of neighbouring templates while they
 // are small and nearly exact, before any sum over bins.  (Staging the rows
 // in LDS as the forward kernels do cost more occupancy than it saved here.)
-template <int W>
+template <int W, bool kDust>
 __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
     SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
     const float* __restrict__ gradw, const float* __restrict__ dcube, int slab_bins,
-    float* __restrict__ vpart, double* __restrict__ apart) {
+    const float* __restrict__ axav, float* __restrict__ vpart, double* __restrict__ apart) {
   __shared__ float red[W][64 * kRedStride];
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const SlabTask t = slab_task<W>(item);
@@ -662,7 +682,10 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
 #pragma unroll 2
       for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
         const Pair xd = ld_pair(tg.pairs, 2 * j);
-        const float gj = ld_at(grow, j), dj = xd.hi;
+        float gj = ld_at(grow, j);
+        // with dust the cube holds sf P_j e_j: every g-sum takes g_j e_j
+        if (kDust && s.av != 0.f) gj *= dust_factor(dust_tau(axav, j), s.av);
+        const float dj = xd.hi;
         const Knots K = knots_of<true>(g, xd.lo, h);
         const float gk = -gj * K.kappa, dk = -dj * K.kappa;
         float w0 = 0.f, w1 = 0.f, u0 = 0.f, u1 = 0.f, s0 = 0.f, s1 = 0.f;
@@ -967,9 +990,14 @@ static void launch_norm(int64_t grid, hipStream_t s, const SspView& g, const rbx
 template <int W>
 static void launch_bwd(int64_t grid, hipStream_t s, const SspView& g, const rbx_target& tg,
                        int64_t n, int64_t items, const StarState* st, const float* gradw,
-                       const float* dcube, int slab_bins, float* vp, double* ap) {
-  hipLaunchKernelGGL(star_bwd_kernel<W>, dim3(grid), dim3(64 * W), 0, s, g, tg, n, items, st,
-                     gradw, dcube, slab_bins, vp, ap);
+                       const float* dcube, int slab_bins, const float* axav, float* vp,
+                       double* ap) {
+  if (axav)
+    hipLaunchKernelGGL((star_bwd_kernel<W, true>), dim3(grid), dim3(64 * W), 0, s, g, tg, n, items,
+                       st, gradw, dcube, slab_bins, axav, vp, ap);
+  else
+    hipLaunchKernelGGL((star_bwd_kernel<W, false>), dim3(grid), dim3(64 * W), 0, s, g, tg, n,
+                       items, st, gradw, dcube, slab_bins, nullptr, vp, ap);
 }
 
 static int validate(const rbx_ssp* ssp, const rbx_target* tg, const rbx_stars* st) {
@@ -1019,10 +1047,13 @@ size_t cube_workspace_impl(int64_t n, int32_t n_keys, int32_t nt, const Tune& t)
 }
 
 int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
-                  const Tune& t, float* cube, void* workspace, size_t workspace_bytes,
-                  hipStream_t s, const char* who) {
+                  const rbx_extinction* ext, const Tune& t, float* cube, void* workspace,
+                  size_t workspace_bytes, hipStream_t s, const char* who) {
   int rc = validate(ssp, target, stars);
   if (rc) return rc;
+  if (ext && ((stars->n > 0 && !ext->av) || !ext->axav))
+    return failf(RBX_EINVAL, "%s: extinction descriptor needs av and axav", who);
+  const float* axav = ext ? ext->axav : nullptr;
   if (!cube) return failf(RBX_EINVAL, "%s: null cube", who);
   const int64_t n = stars->n;
   const int32_t nt = target->n;
@@ -1038,7 +1069,7 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   const int tps = (n_tiles + kSlabs - 1) / kSlabs;
   if (n > 0) {
     hipLaunchKernelGGL(star_prep_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *target,
-                       *stars, L.perm, L.skey, L.state, L.gradw);
+                       *stars, L.perm, L.skey, ext ? ext->av : nullptr, L.state, L.gradw);
     const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
     const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
     switch (t.waves) {
@@ -1058,16 +1089,27 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   if (n > 0 && target->ascending && tpx <= kLaneBins) {
     const int64_t items = int64_t(8) * ((L.max_chunks + 3) / 4);
     const int64_t grid = item_grid(items, t.persist, 256);
-    hipLaunchKernelGGL(cube_accum_staged_kernel<4>, dim3(grid), dim3(256), 0, s, g, *target,
-                       L.state, L.desc, L.chunk_off + ns, tpx * kTile, items, L.partial);
+    if (axav)
+      hipLaunchKernelGGL((cube_accum_staged_kernel<4, true>), dim3(grid), dim3(256), 0, s, g,
+                         *target, L.state, L.desc, L.chunk_off + ns, tpx * kTile, items, axav,
+                         L.partial);
+    else
+      hipLaunchKernelGGL((cube_accum_staged_kernel<4, false>), dim3(grid), dim3(256), 0, s, g,
+                         *target, L.state, L.desc, L.chunk_off + ns, tpx * kTile, items, nullptr,
+                         L.partial);
     if ((rc = check_launch(who))) return rc;
   } else if (n > 0) {
     const int64_t items = int64_t(8) * tpx * L.max_chunks;
     if (items >= (int64_t(1) << 34))
       return failf(RBX_EINVAL, "%s: problem too large for one launch", who);
     const int64_t grid = item_grid(items, t.persist, 64);
-    hipLaunchKernelGGL(cube_accum_kernel, dim3(grid), dim3(64), 0, s, g, *target, L.state,
-                       L.desc, L.chunk_off + ns, n_tiles, tpx, items, L.partial);
+    if (axav)
+      hipLaunchKernelGGL(cube_accum_kernel<true>, dim3(grid), dim3(64), 0, s, g, *target, L.state,
+                         L.desc, L.chunk_off + ns, n_tiles, tpx, items, axav, L.partial);
+    else
+      hipLaunchKernelGGL(cube_accum_kernel<false>, dim3(grid), dim3(64), 0, s, g, *target,
+                         L.state, L.desc, L.chunk_off + ns, n_tiles, tpx, items, nullptr,
+                         L.partial);
     if ((rc = check_launch(who))) return rc;
   }
   combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s);
@@ -1075,11 +1117,14 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
 }
 
 int cube_bwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
-                  const Tune& t, const float* dcube, float* dmass, float* dage, float* dmet,
-                  float* dvlos, void* workspace, size_t workspace_bytes, hipStream_t s,
-                  const char* who) {
+                  const rbx_extinction* ext, const Tune& t, const float* dcube, float* dmass,
+                  float* dage, float* dmet, float* dvlos, void* workspace,
+                  size_t workspace_bytes, hipStream_t s, const char* who) {
   int rc = validate(ssp, target, stars);
   if (rc) return rc;
+  if (ext && !ext->axav)
+    return failf(RBX_EINVAL, "%s: extinction descriptor needs axav", who);
+  const float* axav = ext ? ext->axav : nullptr;
   if (!dcube) return failf(RBX_EINVAL, "%s: null dcube", who);
   const int64_t n = stars->n;
   Carver c(workspace, workspace_bytes);
@@ -1094,10 +1139,10 @@ int cube_bwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
   const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
   switch (t.waves) {
-    case 1: launch_bwd<1>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart); break;
-    case 2: launch_bwd<2>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart); break;
-    case 8: launch_bwd<8>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart); break;
-    default: launch_bwd<4>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, L.vpart, L.apart);
+    case 1: launch_bwd<1>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, axav, L.vpart, L.apart); break;
+    case 2: launch_bwd<2>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, axav, L.vpart, L.apart); break;
+    case 8: launch_bwd<8>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, axav, L.vpart, L.apart); break;
+    default: launch_bwd<4>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, axav, L.vpart, L.apart);
   }
   if ((rc = check_launch(who))) return rc;
   hipLaunchKernelGGL(star_grad_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *stars,
@@ -1134,8 +1179,17 @@ int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* 
                  float* cube, void* workspace, size_t workspace_bytes, void* stream) {
   rbx::Tune t;
   rbx::tune_of(nullptr, &t);
-  return rbx::cube_fwd_impl(ssp, target, stars, t, cube, workspace, workspace_bytes,
+  return rbx::cube_fwd_impl(ssp, target, stars, nullptr, t, cube, workspace, workspace_bytes,
                             rbx::as_stream(stream), "rbx_cube_fwd");
+}
+
+int rbx_cube_fwd_ext(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
+                     const rbx_extinction* ext, float* cube, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+  rbx::Tune t;
+  rbx::tune_of(nullptr, &t);
+  return rbx::cube_fwd_impl(ssp, target, stars, ext, t, cube, workspace, workspace_bytes,
+                            rbx::as_stream(stream), "rbx_cube_fwd_ext");
 }
 
 int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
@@ -1143,8 +1197,19 @@ int rbx_cube_bwd(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* 
                  void* workspace, size_t workspace_bytes, void* stream) {
   rbx::Tune t;
   rbx::tune_of(nullptr, &t);
-  return rbx::cube_bwd_impl(ssp, target, stars, t, dcube, dmass, dage, dmet, dvlos, workspace,
-                            workspace_bytes, rbx::as_stream(stream), "rbx_cube_bwd");
+  return rbx::cube_bwd_impl(ssp, target, stars, nullptr, t, dcube, dmass, dage, dmet, dvlos,
+                            workspace, workspace_bytes, rbx::as_stream(stream), "rbx_cube_bwd");
+}
+
+int rbx_cube_bwd_ext(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
+                     const rbx_extinction* ext, const float* dcube, float* dmass, float* dage,
+                     float* dmet, float* dvlos, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  rbx::Tune t;
+  rbx::tune_of(nullptr, &t);
+  return rbx::cube_bwd_impl(ssp, target, stars, ext, t, dcube, dmass, dage, dmet, dvlos,
+                            workspace, workspace_bytes, rbx::as_stream(stream),
+                            "rbx_cube_bwd_ext");
 }
 
 }  // extern "C"

@@ -17,13 +17,14 @@ int tune_of(const rbx_tuning* t, Tune* out);
 
 size_t cube_workspace_impl(int64_t n, int32_t n_keys, int32_t nt, const Tune& t);
 
+// ext may be NULL (no dust extinction).
 int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
-                  const Tune& t, float* cube, void* workspace, size_t workspace_bytes,
-                  hipStream_t s, const char* who);
+                  const rbx_extinction* ext, const Tune& t, float* cube, void* workspace,
+                  size_t workspace_bytes, hipStream_t s, const char* who);
 
 int cube_bwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
-                  const Tune& t, const float* dcube, float* dmass, float* dage, float* dmet,
-                  float* dvlos, void* workspace, size_t workspace_bytes, hipStream_t s,
-                  const char* who);
+                  const rbx_extinction* ext, const Tune& t, const float* dcube, float* dmass,
+                  float* dage, float* dmet, float* dvlos, void* workspace,
+                  size_t workspace_bytes, hipStream_t s, const char* who);
 
 }  // namespace rbx

@@ -13,8 +13,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (RbxGalaxy, RbxRotation, RbxSsp, RbxStars, RbxTarget, RbxTuning, call, ptr,
-                   stream_handle)
+from ._lib import (RbxExtinction, RbxGalaxy, RbxRotation, RbxSsp, RbxStars, RbxTarget, RbxTuning,
+                   call, ptr, stream_handle)
 
 SPEED_OF_LIGHT = 299792.458  # km/s, rubix/config/rubix_config.yml:8
 VELOCITY_COMPONENT = {"x": 0, "y": 1, "z": 2}
@@ -268,6 +268,7 @@ class CubeContext:
     stars: RbxStars
     workspace: torch.Tensor
     keep: tuple
+    ext: object = None  # RbxExtinction or None
 
 
 def _stars_struct(mass, age, met, velocity, ids, n_spaxel, n_keys, vel_comp, c):
@@ -293,10 +294,13 @@ def _stars_struct(mass, age, met, velocity, ids, n_spaxel, n_keys, vel_comp, c):
 
 
 def cube_forward(ssp: PreparedSSP, target: TargetGrid, mass, age, met, velocity, ids,
-                 n_spaxel: int, n_keys: int, vel_comp: int = 2, c: float = SPEED_OF_LIGHT):
+                 n_spaxel: int, n_keys: int, vel_comp: int = 2, c: float = SPEED_OF_LIGHT,
+                 extinction=None):
     """Fused calculate_spectra -> scale_spectrum_by_mass ->
-    doppler_shift_and_resampling -> calculate_datacube.  Returns the raw cube
-    (n_spaxel, n_target) and the context the adjoint needs."""
+    doppler_shift_and_resampling [-> calculate_extinction] -> calculate_datacube.
+    Returns the raw cube (n_spaxel, n_target) and the context the adjoint
+    needs.  extinction: optional (av per star, axav per target bin) device
+    tensors (rubix/spectra/dust/dust_extinction.py:330-358)."""
     _need_gpu(mass, age, met, velocity, ids)
     mass = _f32(mass, "mass")
     age = _f32(age, "age")
@@ -311,9 +315,19 @@ def cube_forward(ssp: PreparedSSP, target: TargetGrid, mass, age, met, velocity,
     dev = mass.device
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     cube = torch.empty(n_spaxel, target.n, dtype=torch.float32, device=dev)
-    call("rbx_cube_fwd", ssp.struct, target.struct, st, ptr(cube), ptr(ws), wsb, stream_handle(dev))
-    ctx = CubeContext(st, ws, (mass, age, met, velocity, ids))
-    return cube, ctx
+    if extinction is None:
+        call("rbx_cube_fwd", ssp.struct, target.struct, st, ptr(cube), ptr(ws), wsb,
+             stream_handle(dev))
+        return cube, CubeContext(st, ws, (mass, age, met, velocity, ids))
+    av = _f32(extinction[0], "av").reshape(-1)
+    axav = _f32(extinction[1], "axav").reshape(-1)
+    if av.numel() != st.n or axav.numel() != target.n:
+        raise ValueError(f"extinction: av has {av.numel()} entries for {st.n} stars, axav "
+                         f"{axav.numel()} for {target.n} bins")
+    ext = RbxExtinction(ptr(av), ptr(axav))
+    call("rbx_cube_fwd_ext", ssp.struct, target.struct, st, ext, ptr(cube), ptr(ws), wsb,
+         stream_handle(dev))
+    return cube, CubeContext(st, ws, (mass, age, met, velocity, ids, av, axav), ext)
 
 
 def cube_backward(ssp: PreparedSSP, target: TargetGrid, ctx: CubeContext, dcube,
@@ -324,8 +338,14 @@ def cube_backward(ssp: PreparedSSP, target: TargetGrid, ctx: CubeContext, dcube,
     n = ctx.stars.n
     dev = dcube.device
     outs = [torch.empty(n, dtype=torch.float32, device=dev) if k else None for k in need]
-    call("rbx_cube_bwd", ssp.struct, target.struct, ctx.stars, ptr(dcube), *[ptr(o) for o in outs],
-         ptr(ctx.workspace), ctx.workspace.numel(), stream_handle(dev))
+    if ctx.ext is None:
+        call("rbx_cube_bwd", ssp.struct, target.struct, ctx.stars, ptr(dcube),
+             *[ptr(o) for o in outs], ptr(ctx.workspace), ctx.workspace.numel(),
+             stream_handle(dev))
+    else:
+        call("rbx_cube_bwd_ext", ssp.struct, target.struct, ctx.stars, ctx.ext, ptr(dcube),
+             *[ptr(o) for o in outs], ptr(ctx.workspace), ctx.workspace.numel(),
+             stream_handle(dev))
     return tuple(outs)
 
 

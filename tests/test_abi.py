@@ -36,6 +36,7 @@ def test_struct_layouts_match_static_asserts():
     assert ctypes.sizeof(_lib.RbxGalaxy) == 16
     assert ctypes.sizeof(_lib.RbxTuning) == 16
     assert ctypes.sizeof(_lib.RbxNoiseStats) == 16  # noise.hip static_assert
+    assert ctypes.sizeof(_lib.RbxExtinction) == 16
 
 
 def test_version_and_errors():

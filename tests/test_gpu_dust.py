@@ -219,3 +219,49 @@ def test_calc_dusty_ifu_chain_runs_and_differentiates():
     for x in leaves:
         assert x.grad is not None and torch.isfinite(x.grad).all()
     assert float(leaves[0].grad.abs().sum()) > 0
+
+
+def test_fused_dusty_chain_equals_per_step_chain():
+    """calc_dusty_ifu_fused (extinction inside the fused kernels) against
+    calc_dusty_ifu (spectra materialised, then attenuated): cube and every
+    gradient."""
+    w = S.config1(n_stars=3000, seed=9)
+    cfg = S.step_config(w)
+    cfg["data"]["args"]["particle_type"] = ["stars", "gas"]
+    st = w.stars
+    gas = S.make_gas(5000, seed=10)
+    cubes, grads = [], []
+    for chain in ("calc_dusty_ifu", "calc_dusty_ifu_fused"):
+        rd = stars_from_arrays(st["coords"], st["velocity"], st["mass"], st["age"],
+                               st["metallicity"], requires_grad=True, halfmassrad_stars=4.0,
+                               gas=gas)
+        leaves = (rd.stars.mass, rd.stars.age, rd.stars.metallicity, rd.stars.velocity)
+        out = RubixPipeline(cfg, chain).run(rd)
+        cube = out.stars.datacube
+        ((cube * cube).sum()).backward()
+        cubes.append(cube.detach().cpu().numpy())
+        grads.append([x.grad.cpu().numpy() for x in leaves])
+    e = rel_err(cubes[1], cubes[0])
+    print(f"fused vs per-step dusty cube: {e:.3e}")
+    assert e <= 1e-5
+    for name, a, b in zip(("mass", "age", "metallicity", "velocity"), grads[1], grads[0]):
+        eg = rel_err(a, b)
+        print(f"  d/d{name}: {eg:.3e}")
+        assert eg <= 1e-5, name
+
+
+def test_fused_extinction_with_zero_av_is_the_plain_cube():
+    w = cfg1(2000)
+    st = w.stars
+    ssp = ops.PreparedSSP(w.grid["flux"], w.grid["ax_met"], w.grid["ax_age"], w.grid["wave"],
+                          w.geom["z"], w.grid.get("log_axes", False), DEV)
+    tg = ops.TargetGrid(w.geom["wave_target"], DEV)
+    nb = len(w.geom["edges"]) - 1
+    ids = ops.spaxel_assign(_t(st["coords"]), _t(w.geom["edges"]))
+    args = (_t(st["mass"]), _t(st["age"]), _t(st["metallicity"]), _t(st["velocity"]), ids,
+            nb * nb, nb * nb)
+    plain, _ = ops.cube_forward(ssp, tg, *args)
+    axav = ops.extinction_curve("Gordon23", 3.1, _t(w.geom["wave_target"]))
+    zero, _ = ops.cube_forward(ssp, tg, *args,
+                               extinction=(torch.zeros(len(st["mass"]), device=DEV), axav))
+    assert torch.equal(plain, zero)
