@@ -119,11 +119,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("single", "large", "batched", "noise"),
+    ap.add_argument("--workload", choices=("single", "large", "batched", "noise", "dust"),
                     default="single",
                     help="single: 1M stars -> 50x50x3721 (the headline); large: 10M stars -> "
                          "150x150x3721; batched: 64 ragged galaxies in one pass; noise: the "
-                         "apply_noise step on a 50x50x3721 cube")
+                         "apply_noise step on a 50x50x3721 cube; dust: the single workload "
+                         "with 1M gas cells and dust extinction (fused calc_dusty_ifu)")
     ap.add_argument("--n-stars", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
@@ -149,10 +150,13 @@ def main():
         return run_batched(args)
     if args.workload == "noise":
         return run_noise(args)
+    if args.workload == "dust":
+        args.plain_steps = 0
+        return run_single(args, dust=True)
     return run_single(args)
 
 
-def run_single(args):
+def run_single(args, dust=False):
     import numpy as np
     import torch
 
@@ -176,10 +180,28 @@ def run_single(args):
     sl = torch.tensor(geo["lsf_sigma"], device=dev, requires_grad=True)
     nbins = edges.numel() - 1
 
+    ext_of = None
+    if dust:
+        # gas cells of the same disk; A_V from the column of each star's
+        # spaxel, recomputed inside every step (Cardelli89, Rv 3.1)
+        gas = {k: torch.as_tensor(v, device=dev) for k, v in S.make_gas(1_000_000, seed=3).items()}
+        axav = ops.extinction_curve("Cardelli89", 3.1, torch.as_tensor(geo["wave_target"],
+                                                                       device=dev))
+        area = float(((geo["edges"][-1] - geo["edges"][0]) / nb) ** 2)
+
+        def ext_of():
+            gids = ops.spaxel_assign(gas["coords"], edges)
+            sids = ops.spaxel_assign(t["coords"], edges)
+            gext = ops.gas_extinction(gas["mass"], gas["metals"], "broken power law fit", "Z", 3.5,
+                                      area)
+            av = ops.column_av(gas["coords"], gids, gext, t["coords"], sids, nb * nb,
+                               nbins * nbins)
+            return av, axav
+
     def fused(mass, age, met, vel):
         ids = ops.spaxel_assign(t["coords"], edges)
         raw = FusedDatacube.apply(mass, age, met, vel, ids, ssp, tg, nb * nb, nbins * nbins, 2,
-                                  ops.SPEED_OF_LIGHT)
+                                  ops.SPEED_OF_LIGHT, ext_of() if ext_of else None)
         c1 = GaussianPSF.apply(raw.view(nb, nb, nt), sp, geo["psf_size"])
         return GaussianLSF.apply(c1, sl, geo["wave_res"])
 
@@ -226,14 +248,15 @@ def run_single(args):
     step_frac = (fwd_b + adj_b) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
     # dominant launch: the per-star adjoint rbx_cube_bwd (star_bwd_kernel, then
     # the small per-star star_grad_kernel)
-    bwd_ms = per_call.get("rbx_cube_bwd", float("nan"))
+    bwd_name = "rbx_cube_bwd_ext" if dust else "rbx_cube_bwd"
+    bwd_ms = per_call.get(bwd_name, float("nan"))
     bwd_bytes = n * (24 * n_src + 8 * nt)
     achieved = bwd_bytes / (bwd_ms * 1e-3) / 1e9
-    measured = (n, nb, nt, n_src) == (1_000_000, 50, 3721, 5000)
+    measured = (n, nb, nt, n_src) == (1_000_000, 50, 3721, 5000) and not dust
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": TRAFFIC_BWD_1M_BYTES if measured else None,
-                "kernel": "rbx_cube_bwd (star_bwd_kernel + star_grad_kernel)",
+                "kernel": f"{bwd_name} (star_bwd_kernel + star_grad_kernel)",
                 "kernel_ms": round(bwd_ms, 4), "algorithmic_bytes_per_launch": bwd_bytes,
                 "step_effective_frac": round(step_frac, 4)}
     split = {"fwd_ms_median": round(float(np.median(fwd_ms)), 4),
@@ -256,19 +279,24 @@ def run_single(args):
         log("plain:", json.dumps(plain_res))
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not dust:
         threads = len(os.sched_getaffinity(0))
         rate, dt = cpu_baseline(w, args.cpu_sample, threads)
         cpu = {"value": rate, "unit": "star·λ-bins/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_sample} stars of the same workload (full SSP grid, 50x50x3721 cube, "
                          f"PSF + LSF), float32 torch-CPU restatement of the plain per-step path, "
                          f"forward + autograd adjoint, one pass {dt:.1f}s on {_model_name()}"}
+    workload = (f"single galaxy, {n} stars -> {nb}x{nb}x{nt} cube, SSP 12x100x{n_src}, "
+                "PSF 15x15 sigma 4 px, LSF sigma 1.0 A; fused forward + adjoint of chi^2")
+    if dust:
+        workload += ("; dust extinction from 1000000 gas cells (Cardelli89, Rv 3.1, broken power "
+                     "law D/G, A_V recomputed each step)")
     res = {
-        "metric": METRIC, "value": value, "unit": "star·λ-bins/s", "n_gpus": 1,
+        "metric": METRIC + (" (with dust extinction)" if dust else ""), "value": value,
+        "unit": "star·λ-bins/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"single galaxy, {n} stars -> {nb}x{nb}x{nt} cube, SSP 12x100x{n_src}, "
-                               "PSF 15x15 sigma 4 px, LSF sigma 1.0 A; fused forward + adjoint of chi^2",
+        "config": {"workload": workload,
                    "n_stars": n, "n_spaxel": nb * nb, "n_target": nt, "n_source": n_src},
         "roofline": roofline, "cpu_baseline": cpu, "split": split, "plain": plain_res,
         "per_call_ms": {k: round(v, 4) for k, v in per_call.items()},
