@@ -35,6 +35,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "rbx_common.h"
 #include "ssp_device.h"
 #include "cube_impl.h"
@@ -280,7 +282,11 @@ struct WinKnots {
   float kappa;  // as Knots::kappa (only when requested)
 };
 
-template <bool kKappa>
+// kInterior: the window lies strictly inside the grid (k_lo >= 1 and
+// k_lo + nk < n_wave), so no segment of it touches knot 0 or n_wave - 1:
+// the grid-end bounds of the correction and the left / right clamps cannot
+// apply and are left out (same result, fewer operations per bin).
+template <bool kKappa, bool kInterior = false>
 __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, int k_lo, int nk,
                                      const float* lam) {
   const int n = g.n_wave;
@@ -296,7 +302,11 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
   float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
   // the global segment [li + k_lo, li + k_lo + 1] moves within [0, n - 1]
   const int li_max = nk - 2, li_min = 0;
-  bool step = (x >= wr && li < li_max && li + k_lo + 1 < n - 1) || (x < wl && li > li_min && li + k_lo > 0);
+  auto needs_step = [&](int l, float lo, float hi) {
+    if (kInterior) return (x >= hi && l < li_max) || (x < lo && l > li_min);
+    return (x >= hi && l < li_max && l + k_lo + 1 < n - 1) || (x < lo && l > li_min && l + k_lo > 0);
+  };
+  bool step = needs_step(li, wl, wr);
   if (__ballot(step)) {
     while (step) {
       li += x >= wr ? 1 : -1;
@@ -304,7 +314,7 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
       l1 = lam[li + 1];
       wl = mul_nocontract(l0, h.a);
       wr = mul_nocontract(l1, h.a);
-      step = (x >= wr && li < li_max && li + k_lo + 1 < n - 1) || (x < wl && li > li_min && li + k_lo > 0);
+      step = needs_step(li, wl, wr);
     }
   }
   const int i = li + k_lo + 1;  // global right knot
@@ -312,8 +322,8 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
   const float r = __builtin_amdgcn_rcpf(dx);
   const bool flat = !(dx > 0.f);
   const float th = flat ? 0.f : (x - wl) * r;
-  const bool left = (i == 1) && (x < wl);
-  const bool right = (i == n - 1) && (x > wr);
+  const bool left = !kInterior && (i == 1) && (x < wl);
+  const bool right = !kInterior && (i == n - 1) && (x > wr);
   WinKnots K;
   K.li = li;
   K.theta = left ? 0.f : (right ? 1.f : th);
@@ -390,13 +400,17 @@ __global__ void __launch_bounds__(64 * W) star_norm_kernel(SspView g, rbx_target
           F[k] = f;
         }
         wave_lds_sync();
+        auto bins = [&](auto interior) {
 #pragma unroll 2
-        for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
-          const Pair xd = ld_pair(tg.pairs, 2 * j);
-          const WinKnots K = win_knots<false>(g, xd.lo, h, k_lo, nk, lam);
-          const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
-          acc += (S0 + K.theta * (S1 - S0)) * xd.hi;
-        }
+          for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
+            const Pair xd = ld_pair(tg.pairs, 2 * j);
+            const WinKnots K = win_knots<false, decltype(interior)::value>(g, xd.lo, h, k_lo, nk, lam);
+            const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
+            acc += (S0 + K.theta * (S1 - S0)) * xd.hi;
+          }
+        };
+        if (k_lo >= 1 && k_lo + nk < g.n_wave) bins(std::true_type{});
+        else bins(std::false_type{});
         wave_lds_sync();
       } else {
 #pragma unroll 2
@@ -564,16 +578,21 @@ __global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
           F[k] = f;
         }
         wave_lds_sync();
+        auto bins = [&](auto interior) {
 #pragma unroll
-        for (int t = 0; t < kLaneBins; ++t) {
-          if (j0 + lane + 64u * t < j1) {
-            const WinKnots K = win_knots<false>(g, x[t], h, k_lo, nk, lam);
-            const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
-            float v = (S0 + K.theta * (S1 - S0)) * s.sf;
-            if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
-            acc[t] += double(v);
+          for (int t = 0; t < kLaneBins; ++t) {
+            if (j0 + lane + 64u * t < j1) {
+              const WinKnots K =
+                  win_knots<false, decltype(interior)::value>(g, x[t], h, k_lo, nk, lam);
+              const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
+              float v = (S0 + K.theta * (S1 - S0)) * s.sf;
+              if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
+              acc[t] += double(v);
+            }
           }
-        }
+        };
+        if (k_lo >= 1 && k_lo + nk < g.n_wave) bins(std::true_type{});
+        else bins(std::false_type{});
         wave_lds_sync();
       } else {
 #pragma unroll
