@@ -332,14 +332,37 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
   return K;
 }
 
+// Locator guess of the left knot of x's segment: k0 of x's cell plus the
+// next-knot test on y = x / a.  With at most one knot per cell
+// (locate_multi == 0) it is within 3 knots of the exact segment: one cell
+// of rounding in the cell index, one knot of rounding in y, one of the
+// shifted knots' rounding.
+__device__ inline int locate_guess(const SspView& g, float x, const Shift& h) {
+  float fm = fmaf(x, h.c1, h.c0);
+  fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
+  const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
+  const int kn = k0 + 1 < g.n_wave - 1 ? k0 + 1 : g.n_wave - 1;
+  return k0 + (ld_at(g.wave, uint32_t(kn < 0 ? 0 : kn)) <= x * h.inv_a ? 1 : 0);
+}
+
 // Knot window [k_lo, k_lo + nk) holding every segment of the bins [j0, j1)
-// (ascending targets): from the first bin's left knot to the last bin's
-// right knot.
+// (ascending targets).  With a fine locator: the guesses of the first and
+// last bin widened by the guess's error bound (3 knots left, 4 right); the
+// bins' exact search then runs inside it.  With a coarse locator: from the
+// first bin's exact left knot to the last bin's exact right knot.
 __device__ inline void slab_window(const SspView& g, const rbx_target& tg, uint32_t j0,
                                    uint32_t j1, const Shift& h, int* k_lo, int* nk) {
   const float xf = ld_at(tg.wave, j0), xl = ld_at(tg.wave, j1 - 1);
-  const int lo = int(knots_of<false>(g, xf, h).i0);
-  const int hi = int(knots_of<false>(g, xl, h).i0) + 1;
+  int lo, hi;
+  if (!g.loc_multi) {
+    lo = locate_guess(g, xf, h) - 3;
+    hi = locate_guess(g, xl, h) + 4;
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > g.n_wave - 1 ? g.n_wave - 1 : hi;
+  } else {
+    lo = int(knots_of<false>(g, xf, h).i0);
+    hi = int(knots_of<false>(g, xl, h).i0) + 1;
+  }
   *k_lo = __builtin_amdgcn_readfirstlane(lo);
   *nk = __builtin_amdgcn_readfirstlane(hi - lo + 1);
 }
@@ -741,6 +764,198 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
   }
 }
 
+// One bin of star_bwd_kernel's sums, from gathered template pairs.
+struct BwdRows {
+  float gu[4], gs[4];
+};
+
+__device__ inline void bwd_bin_gather(const SspView& g, const Rows& R, const StarState& s,
+                                      const BwdRows& B, const Shift& h, float x, float gj,
+                                      float dj, float v[7]) {
+  const Knots K = knots_of<true>(g, x, h);
+  const float gk = -gj * K.kappa, dk = -dj * K.kappa;
+  float w0 = 0.f, w1 = 0.f, u0 = 0.f, u1 = 0.f, s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const Pair f = ld_pair(R.f[c], K.i0);
+    w0 += s.w[c] * f.lo;
+    w1 += s.w[c] * f.hi;
+    u0 += B.gu[c] * f.lo;
+    u1 += B.gu[c] * f.hi;
+    s0 += B.gs[c] * f.lo;
+    s1 += B.gs[c] * f.hi;
+  }
+  const float dw = w1 - w0;
+  const float Rw = w0 + K.theta * dw;
+  const float Ru = u0 + K.theta * (u1 - u0);
+  const float Rs = s0 + K.theta * (s1 - s0);
+  v[0] += gj * Rw;
+  v[1] += gj * Ru;
+  v[2] += gj * Rs;
+  v[3] += dj * Ru;
+  v[4] += dj * Rs;
+  v[5] += gk * dw;
+  v[6] += dk * dw;
+}
+
+// star_bwd_kernel for ascending targets with a register window: for each
+// group of 64 consecutive bins (one per lane) the knots the group can touch,
+// [k0(first bin) - 1, k0(last bin) + 2] from the compact locator, are
+// loaded one per lane with coalesced loads (the knot and its four corner
+// values, combined into R_w, R_u, R_s in the lane), and every bin takes its
+// segment's two knots from the owning lanes by cross-lane permutes instead
+// of gathering the template pairs.  The segment is the exact one of
+// win_knots, searched on the permuted knots with a wave-uniform correction
+// loop.  A group whose knots do not fit one lane each, or where a bin's
+// segment would leave the window, is done with the gathers instead.
+template <int W, bool kDust>
+__global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
+    SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
+    const float* __restrict__ gradw, const float* __restrict__ dcube, int slab_bins,
+    const float* __restrict__ axav, float* __restrict__ vpart, double* __restrict__ apart) {
+  __shared__ float red[W][64 * kRedStride];
+  const int n_w = g.n_wave;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const SlabTask t = slab_task<W>(item);
+    const bool live = t.r < n;
+    float v[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) v[q] = 0.f;
+    StarState s;
+    s.row = -1;
+    if (live) s = state[t.r];
+    if (s.row >= 0) {
+      const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
+      const float* grow = dcube + int64_t(s.spaxel) * tg.n;
+      const Shift h = shift_of(g, s.a, s.inv_a);
+      BwdRows B;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        B.gu[c] = gradw[t.r * 8 + c];
+        B.gs[c] = gradw[t.r * 8 + 4 + c];
+      }
+      const uint32_t j0 = uint32_t(t.slab * slab_bins);
+      const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
+      // interior slab: every bin's segment has knots in [1, n - 2] (x ascending
+      // from x_first >= w_1 to x_last < w_{n-2}), so no grid-end bound or
+      // clamp can apply to it
+      bool interior = false;
+      if (j0 < j1) {
+        const float w1 = mul_nocontract(ld_at(g.wave, 1u), h.a);
+        const float wn = mul_nocontract(ld_at(g.wave, uint32_t(n_w - 2)), h.a);
+        interior = n_w >= 4 && ld_at(tg.wave, j0) >= w1 && ld_at(tg.wave, j1 - 1) < wn;
+      }
+      auto groups = [&](auto interior_c) {
+        constexpr bool kIn = decltype(interior_c)::value;
+      for (uint32_t jb = j0; jb < j1; jb += 64) {
+        const uint32_t j = jb + uint32_t(t.lane);
+        const bool valid = j < j1;
+        const uint32_t jc = valid ? j : j1 - 1;
+        const Pair xd = ld_pair(tg.pairs, 2 * jc);
+        const float x = xd.lo;
+        float gj = valid ? ld_at(grow, jc) : 0.f;
+        if (kDust && s.av != 0.f) gj *= dust_factor(dust_tau(axav, jc), s.av);
+        const float dj = valid ? xd.hi : 0.f;
+        // locator guess (compact copy: k0 = knots <= cell start - 1)
+        float fm = fmaf(x, h.c1, h.c0);
+        fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
+        const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
+        const int last = int(j1 - jb < 64u ? j1 - jb - 1 : 63u);
+        int wb = __builtin_amdgcn_readfirstlane(k0) - 1;
+        int we = __builtin_amdgcn_readlane(k0, last) + 2;
+        wb = wb < 0 ? 0 : wb;
+        we = we > n_w - 1 ? n_w - 1 : we;
+        const int span = we - wb + 1;
+        bool gather = span > 64 || span < 2;
+        if (!gather) {
+          // this lane's knot of the window and its combined rows
+          const int kl = wb + t.lane;
+          const uint32_t kk = uint32_t(kl < we ? kl : we);
+          const float lamk = ld_at(g.wave, kk);
+          float rw = 0.f, ru = 0.f, rs = 0.f;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float f = ld_at(R.f[c], kk);
+            rw += s.w[c] * f;
+            ru += B.gu[c] * f;
+            rs += B.gs[c] * f;
+          }
+          // exact segment on the window's knots (as win_knots)
+          const float y = x * h.inv_a;
+          int li = k0 - wb;
+          li = li < 0 ? 0 : (li > span - 2 ? span - 2 : li);
+          const float ln = __shfl(lamk, li + 1, 64);
+          if (li + 1 < span - 1 && ln <= y) ++li;
+          float l0 = __shfl(lamk, li, 64), l1 = __shfl(lamk, li + 1, 64);
+          float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
+          auto needs_step = [&](int l, float lo, float hi) {
+            if (kIn) return (x >= hi && l < span - 2) || (x < lo && l > 0);
+            return (x >= hi && l < span - 2 && l + wb + 1 < n_w - 1) ||
+                   (x < lo && l > 0 && l + wb > 0);
+          };
+          bool step = needs_step(li, wl, wr);
+          while (__ballot(step)) {
+            const int nl = step ? li + (x >= wr ? 1 : -1) : li;
+            const float a0 = __shfl(lamk, nl, 64), a1 = __shfl(lamk, nl + 1, 64);
+            if (step) {
+              li = nl;
+              l0 = a0;
+              l1 = a1;
+              wl = mul_nocontract(l0, h.a);
+              wr = mul_nocontract(l1, h.a);
+            }
+            step = needs_step(li, wl, wr);
+          }
+          // a segment pinned at a window edge that is not a grid end: the
+          // window was too narrow for this lane
+          const bool off =
+              valid && (kIn ? ((x >= wr && li == span - 2) || (x < wl && li == 0))
+                            : ((x >= wr && li == span - 2 && li + wb + 1 < n_w - 1) ||
+                               (x < wl && li == 0 && li + wb > 0)));
+          gather = __ballot(off) != 0;
+          if (!gather) {
+            const int i = li + wb + 1;
+            const float dx = wr - wl;
+            const float r = __builtin_amdgcn_rcpf(dx);
+            const bool flat = !(dx > 0.f);
+            const float th0 = flat ? 0.f : (x - wl) * r;
+            const bool left = !kIn && (i == 1) && (x < wl);
+            const bool right = !kIn && (i == n_w - 1) && (x > wr);
+            const float th = left ? 0.f : (right ? 1.f : th0);
+            const float kap = (left || right || flat) ? 0.f : (l0 + th0 * (l1 - l0)) * r;
+            const float w0 = __shfl(rw, li, 64), w1 = __shfl(rw, li + 1, 64);
+            const float u0 = __shfl(ru, li, 64), u1 = __shfl(ru, li + 1, 64);
+            const float s0 = __shfl(rs, li, 64), s1 = __shfl(rs, li + 1, 64);
+            const float gk = -gj * kap, dk = -dj * kap;
+            const float dw = w1 - w0;
+            const float Rwv = w0 + th * dw;
+            const float Ruv = u0 + th * (u1 - u0);
+            const float Rsv = s0 + th * (s1 - s0);
+            v[0] += gj * Rwv;
+            v[1] += gj * Ruv;
+            v[2] += gj * Rsv;
+            v[3] += dj * Ruv;
+            v[4] += dj * Rsv;
+            v[5] += gk * dw;
+            v[6] += dk * dw;
+          }
+        }
+        if (gather && valid) bwd_bin_gather(g, R, s, B, h, x, gj, dj, v);
+      }
+      };
+      if (interior) groups(std::true_type{});
+      else groups(std::false_type{});
+    }
+    const double tot = lds_sum<7>(v, red[t.wave], t.lane);
+    const int q = t.lane >> 2;
+    if (live && (t.lane & 3) == 0 && q < 7) {
+      const int64_t base = (t.r * kSlabs + t.slab) * 8;
+      if (q < 5) vpart[base + q] = float(tot);
+      else apart[base + q - 5] = tot;
+    }
+  }
+}
+
 // Per star: add the slab sums in slab order and form every gradient in
 // closed form.  With P_j = m sum_c w_c R_c(j) and sf = T / Nn:
 //   sum_j g_j dP_j/dtheta = m Pg_theta, dNn/dtheta = m Pd_theta,
@@ -1011,6 +1226,15 @@ static void launch_bwd(int64_t grid, hipStream_t s, const SspView& g, const rbx_
                        int64_t n, int64_t items, const StarState* st, const float* gradw,
                        const float* dcube, int slab_bins, const float* axav, float* vp,
                        double* ap) {
+  if (tg.ascending) {
+    if (axav)
+      hipLaunchKernelGGL((star_bwd_reg_kernel<W, true>), dim3(grid), dim3(64 * W), 0, s, g, tg, n,
+                         items, st, gradw, dcube, slab_bins, axav, vp, ap);
+    else
+      hipLaunchKernelGGL((star_bwd_reg_kernel<W, false>), dim3(grid), dim3(64 * W), 0, s, g, tg,
+                         n, items, st, gradw, dcube, slab_bins, nullptr, vp, ap);
+    return;
+  }
   if (axav)
     hipLaunchKernelGGL((star_bwd_kernel<W, true>), dim3(grid), dim3(64 * W), 0, s, g, tg, n, items,
                        st, gradw, dcube, slab_bins, axav, vp, ap);

@@ -42,6 +42,7 @@ struct SspView {
   int n_met, n_age, n_wave, n_loc;
   float loc_x0, loc_inv_h;
   int log_axes;
+  int loc_multi;  // cells may hold several knots (coarse locator)
 };
 
 inline SspView view_of(const rbx_ssp& s) {
@@ -50,6 +51,7 @@ inline SspView view_of(const rbx_ssp& s) {
   v.cum = s.cum; v.loc = s.locate; v.n_met = s.n_met; v.n_age = s.n_age;
   v.n_wave = s.n_wave; v.n_loc = s.n_locate; v.loc_x0 = s.locate_x0;
   v.loc_inv_h = s.locate_inv_h; v.log_axes = s.log_axes;
+  v.loc_multi = s.locate_multi;
   return v;
 }
 
