@@ -24,11 +24,11 @@ if ROOT not in sys.path:
 
 METRIC = "star·λ-bins/s fwd+adjoint, 1M stars→50²×3721 cube; % HBM roofline vs plain"
 HBM_PEAK_GBS = 8000.0
-# Physical HBM bytes of one rbx_cube_bwd launch (star_bwd_kernel +
+# Physical HBM bytes of one rbx_cube_bwd launch (star_bwd_reg_kernel +
 # star_grad_kernel) at 1M stars -> 50x50x3721 with the 12x100x5000 grid:
 # rocprofv3 FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), the factors from a 1 GiB
 # device copy (bench.py --calibrate-copy); profiles/hbm_traffic_pmc_1M_50x50x3721.md.
-TRAFFIC_BWD_1M_BYTES = 3.227e9
+TRAFFIC_BWD_1M_BYTES = 3.552e9
 
 
 def log(*a):
@@ -256,7 +256,7 @@ def run_single(args, dust=False):
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": TRAFFIC_BWD_1M_BYTES if measured else None,
-                "kernel": f"{bwd_name} (star_bwd_kernel + star_grad_kernel)",
+                "kernel": f"{bwd_name} (star_bwd_reg_kernel + star_grad_kernel)",
                 "kernel_ms": round(bwd_ms, 4), "algorithmic_bytes_per_launch": bwd_bytes,
                 "step_effective_frac": round(step_frac, 4)}
     split = {"fwd_ms_median": round(float(np.median(fwd_ms)), 4),
