@@ -290,18 +290,19 @@ template <bool kKappa, bool kInterior = false>
 __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, int k_lo, int nk,
                                      const float* lam) {
   const int n = g.n_wave;
-  float fm = fmaf(x, h.c1, h.c0);
-  fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
+  const float fm = __builtin_amdgcn_fmed3f(fmaf(x, h.c1, h.c0), 0.f, float(g.n_loc - 1));
   // compact locator copy (4 bytes per cell): k0 = knots <= cell start - 1
   const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
   const float y = x * h.inv_a;
+  const int li_max = nk - 2, li_min = 0;
   int li = k0 - k_lo;
-  li = li < 0 ? 0 : (li > nk - 2 ? nk - 2 : li);
-  if (li + 1 < nk - 1 && lam[li + 1] <= y) ++li;
+  li = li < 0 ? 0 : (li > li_max ? li_max : li);
+  // next-knot test, kept inside [0, nk - 2] (lam[li + 1] is in the window)
+  li += lam[li + 1] <= y ? 1 : 0;
+  li = li > li_max ? li_max : li;
   float l0 = lam[li], l1 = lam[li + 1];
   float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
   // the global segment [li + k_lo, li + k_lo + 1] moves within [0, n - 1]
-  const int li_max = nk - 2, li_min = 0;
   auto needs_step = [&](int l, float lo, float hi) {
     if (kInterior) return (x >= hi && l < li_max) || (x < lo && l > li_min);
     return (x >= hi && l < li_max && l + k_lo + 1 < n - 1) || (x < lo && l > li_min && l + k_lo > 0);
