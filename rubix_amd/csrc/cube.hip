@@ -858,8 +858,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         if (kDust && s.av != 0.f) gj *= dust_factor(dust_tau(axav, jc), s.av);
         const float dj = valid ? xd.hi : 0.f;
         // locator guess (compact copy: k0 = knots <= cell start - 1)
-        float fm = fmaf(x, h.c1, h.c0);
-        fm = fminf(fmaxf(fm, 0.f), float(g.n_loc - 1));
+        const float fm = __builtin_amdgcn_fmed3f(fmaf(x, h.c1, h.c0), 0.f, float(g.n_loc - 1));
         const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
         const int last = int(j1 - jb < 64u ? j1 - jb - 1 : 63u);
         int wb = __builtin_amdgcn_readfirstlane(k0) - 1;
@@ -886,7 +885,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
           int li = k0 - wb;
           li = li < 0 ? 0 : (li > span - 2 ? span - 2 : li);
           const float ln = __shfl(lamk, li + 1, 64);
-          if (li + 1 < span - 1 && ln <= y) ++li;
+          li += ln <= y ? 1 : 0;
+          li = li > span - 2 ? span - 2 : li;
           float l0 = __shfl(lamk, li, 64), l1 = __shfl(lamk, li + 1, 64);
           float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
           auto needs_step = [&](int l, float lo, float hi) {
