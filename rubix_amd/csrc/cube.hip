@@ -283,9 +283,15 @@ struct WinKnots {
 };
 
 // kInterior: the window lies strictly inside the grid (k_lo >= 1 and
-// k_lo + nk < n_wave), so no segment of it touches knot 0 or n_wave - 1:
+// k_lo + nk < n_wave) and was built by slab_window from locator guesses
+// (locate_multi == 0), so no segment of it touches knot 0 or n_wave - 1:
 // the grid-end bounds of the correction and the left / right clamps cannot
-// apply and are left out (same result, fewer operations per bin).
+// apply and are left out (same result, fewer operations per bin).  The
+// guess needs no clamp either: k0 is monotone in x, so for the slab's bins
+// k0(x_first) <= k0 <= k0(x_last), and the window's margins (3 knots below
+// the first guess, 4 above the last) put li = k0 - k_lo in [2, nk - 5].
+// The search then ends on the exact segment w_l <= x < w_r, whose width
+// is positive.
 template <bool kKappa, bool kInterior = false>
 __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, int k_lo, int nk,
                                      const float* lam) {
@@ -296,10 +302,10 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
   const float y = x * h.inv_a;
   const int li_max = nk - 2, li_min = 0;
   int li = k0 - k_lo;
-  li = li < 0 ? 0 : (li > li_max ? li_max : li);
+  if (!kInterior) li = li < 0 ? 0 : (li > li_max ? li_max : li);
   // next-knot test, kept inside [0, nk - 2] (lam[li + 1] is in the window)
   li += lam[li + 1] <= y ? 1 : 0;
-  li = li > li_max ? li_max : li;
+  if (!kInterior) li = li > li_max ? li_max : li;
   float l0 = lam[li], l1 = lam[li + 1];
   float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
   // the global segment [li + k_lo, li + k_lo + 1] moves within [0, n - 1]
@@ -321,7 +327,7 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
   const int i = li + k_lo + 1;  // global right knot
   const float dx = wr - wl;
   const float r = __builtin_amdgcn_rcpf(dx);
-  const bool flat = !(dx > 0.f);
+  const bool flat = !kInterior && !(dx > 0.f);
   const float th = flat ? 0.f : (x - wl) * r;
   const bool left = !kInterior && (i == 1) && (x < wl);
   const bool right = !kInterior && (i == n - 1) && (x > wr);
@@ -433,7 +439,7 @@ __global__ void __launch_bounds__(64 * W) star_norm_kernel(SspView g, rbx_target
             acc += (S0 + K.theta * (S1 - S0)) * xd.hi;
           }
         };
-        if (k_lo >= 1 && k_lo + nk < g.n_wave) bins(std::true_type{});
+        if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) bins(std::true_type{});
         else bins(std::false_type{});
         wave_lds_sync();
       } else {
@@ -615,7 +621,7 @@ __global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
             }
           }
         };
-        if (k_lo >= 1 && k_lo + nk < g.n_wave) bins(std::true_type{});
+        if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) bins(std::true_type{});
         else bins(std::false_type{});
         wave_lds_sync();
       } else {
