@@ -805,6 +805,11 @@ __device__ inline void bwd_bin_gather(const SspView& g, const Rows& R, const Sta
   v[6] += dk * dw;
 }
 
+// v of lane l (0 <= l < 64) by one ds_bpermute (no lane wrapping needed).
+__device__ inline float lane_of(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(l << 2, __float_as_int(v)));
+}
+
 // star_bwd_kernel for ascending targets with a register window: for each
 // group of 64 consecutive bins (one per lane) the knots the group can touch,
 // [k0(first bin) - 1, k0(last bin) + 2] from the compact locator, are
@@ -890,10 +895,10 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
           const float y = x * h.inv_a;
           int li = k0 - wb;
           li = li < 0 ? 0 : (li > span - 2 ? span - 2 : li);
-          const float ln = __shfl(lamk, li + 1, 64);
+          const float ln = lane_of(lamk, li + 1);
           li += ln <= y ? 1 : 0;
           li = li > span - 2 ? span - 2 : li;
-          float l0 = __shfl(lamk, li, 64), l1 = __shfl(lamk, li + 1, 64);
+          float l0 = lane_of(lamk, li), l1 = lane_of(lamk, li + 1);
           float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
           auto needs_step = [&](int l, float lo, float hi) {
             if (kIn) return (x >= hi && l < span - 2) || (x < lo && l > 0);
@@ -901,17 +906,19 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
                    (x < lo && l > 0 && l + wb > 0);
           };
           bool step = needs_step(li, wl, wr);
-          while (__ballot(step)) {
-            const int nl = step ? li + (x >= wr ? 1 : -1) : li;
-            const float a0 = __shfl(lamk, nl, 64), a1 = __shfl(lamk, nl + 1, 64);
-            if (step) {
-              li = nl;
-              l0 = a0;
-              l1 = a1;
-              wl = mul_nocontract(l0, h.a);
-              wr = mul_nocontract(l1, h.a);
-            }
-            step = needs_step(li, wl, wr);
+          if (__ballot(step)) {  // rare: the common path carries no loop state
+            do {
+              const int nl = step ? li + (x >= wr ? 1 : -1) : li;
+              const float a0 = lane_of(lamk, nl), a1 = lane_of(lamk, nl + 1);
+              if (step) {
+                li = nl;
+                l0 = a0;
+                l1 = a1;
+                wl = mul_nocontract(l0, h.a);
+                wr = mul_nocontract(l1, h.a);
+              }
+              step = needs_step(li, wl, wr);
+            } while (__ballot(step));
           }
           // a segment pinned at a window edge that is not a grid end: the
           // window was too narrow for this lane
@@ -924,15 +931,17 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
             const int i = li + wb + 1;
             const float dx = wr - wl;
             const float r = __builtin_amdgcn_rcpf(dx);
-            const bool flat = !(dx > 0.f);
+            // interior: a lane that is not off ends on w_l <= x < w_r (the
+            // lanes past the slab repeat its last bin), so dx > 0
+            const bool flat = !kIn && !(dx > 0.f);
             const float th0 = flat ? 0.f : (x - wl) * r;
             const bool left = !kIn && (i == 1) && (x < wl);
             const bool right = !kIn && (i == n_w - 1) && (x > wr);
             const float th = left ? 0.f : (right ? 1.f : th0);
             const float kap = (left || right || flat) ? 0.f : (l0 + th0 * (l1 - l0)) * r;
-            const float w0 = __shfl(rw, li, 64), w1 = __shfl(rw, li + 1, 64);
-            const float u0 = __shfl(ru, li, 64), u1 = __shfl(ru, li + 1, 64);
-            const float s0 = __shfl(rs, li, 64), s1 = __shfl(rs, li + 1, 64);
+            const float w0 = lane_of(rw, li), w1 = lane_of(rw, li + 1);
+            const float u0 = lane_of(ru, li), u1 = lane_of(ru, li + 1);
+            const float s0 = lane_of(rs, li), s1 = lane_of(rs, li + 1);
             const float gk = -gj * kap, dk = -dj * kap;
             const float dw = w1 - w0;
             const float Rwv = w0 + th * dw;
