@@ -11,6 +11,7 @@ tests monkeypatch jax.device_count the same way).
 from __future__ import annotations
 
 import logging
+import os
 from dataclasses import dataclass, fields
 from typing import Callable, Optional
 
@@ -18,6 +19,7 @@ import numpy as np
 import torch
 
 from ..logger import get_logger
+from ..utils import load_galaxy_data, read_yaml
 
 
 def device_count() -> int:
@@ -30,6 +32,9 @@ class Galaxy:
     redshift: Optional[object] = None
     center: Optional[object] = None
     halfmassrad_stars: Optional[object] = None
+    redshift_unit: Optional[str] = None
+    center_unit: Optional[str] = None
+    halfmassrad_stars_unit: Optional[str] = None
 
 
 @dataclass
@@ -147,3 +152,121 @@ def stars_from_arrays(coords, velocity, mass, age, metallicity, device="cuda",
                      stars=StarsData(coords=plain(coords), velocity=t(velocity), mass=t(mass),
                                      age=t(age), metallicity=t(metallicity)),
                      gas=gas_data)
+
+
+def _like_jnp_array(value, device) -> torch.Tensor:
+    """A loaded array as jnp.array makes it with 64-bit types disabled
+    (JAX's default): float64 -> float32, int64 -> int32, uint64 -> uint32."""
+    a = np.asarray(value)
+    narrow = {np.dtype(np.float64): np.float32, np.dtype(np.int64): np.int32,
+              np.dtype(np.uint64): np.uint32}
+    if a.dtype in narrow:
+        a = a.astype(narrow[a.dtype])
+    return torch.as_tensor(np.ascontiguousarray(a), device=device)
+
+
+def _midpoint_median(v: torch.Tensor) -> torch.Tensor:
+    """Median along axis 0 as jnp.median computes it: the middle value, or
+    the midpoint (a + b) / 2 of the two middle values; NaN when empty."""
+    n = v.shape[0]
+    if n == 0:
+        return torch.full(v.shape[1:], float("nan"), dtype=v.dtype, device=v.device)
+    s = torch.sort(v, dim=0).values
+    if n % 2:
+        return s[n // 2]
+    return (s[n // 2 - 1] + s[n // 2]) / 2
+
+
+def center_particles(rubixdata: RubixData, key: str) -> RubixData:
+    """rubix/galaxy/alignment.py:10-62: subtract the galaxy center from the
+    particle coordinates and the median velocity of the particles within
+    10 kpc of it from their velocities.  The center (a float64 array in the
+    file) meets the float32 particle arrays in float32, as in the
+    reference."""
+    if key == "stars":
+        pc, pv = rubixdata.stars.coords, rubixdata.stars.velocity
+    elif key == "gas":
+        pc, pv = rubixdata.gas.coords, rubixdata.gas.velocity
+    c = torch.as_tensor(np.asarray(rubixdata.galaxy.center), device=pc.device).to(pc.dtype)
+    lo, hi = pc.min(dim=0).values, pc.max(dim=0).values
+    if not bool(torch.all((c >= lo[:3]) & (c <= hi[:3]))):
+        raise ValueError("Center is not within the bounds of the galaxy")
+    mask = torch.linalg.norm(pc - c, dim=1) < 10
+    central_velocity = _midpoint_median(pv[mask])
+    if key == "stars":
+        rubixdata.stars.coords = pc - c
+        rubixdata.stars.velocity = pv - central_velocity
+    elif key == "gas":
+        rubixdata.gas.coords = pc - c
+        rubixdata.gas.velocity = pv - central_velocity
+    return rubixdata
+
+
+def prepare_input(config) -> RubixData:
+    """rubix/core/data.py:491-604: RubixData from
+    config["output_path"]/rubix_galaxy.h5 (galaxy fields and units, every
+    particle field of the requested types as a device tensor with its unit,
+    centred; the optional random subset with the reference's seed and its
+    index rule, including its use of the star count when subsetting gas)."""
+    if isinstance(config, str):
+        config = read_yaml(config)
+    logger = get_logger(config["logger"] if "logger" in config else None)
+    file_path = os.path.join(config["output_path"], "rubix_galaxy.h5")
+    data, units = load_galaxy_data(file_path)
+    device = torch.device(config.get("device", "cuda"))
+
+    rubixdata = RubixData(Galaxy(), StarsData(), GasData())
+    rubixdata.galaxy.redshift = data["redshift"]
+    rubixdata.galaxy.redshift_unit = units["galaxy"]["redshift"]
+    rubixdata.galaxy.center = data["subhalo_center"]
+    rubixdata.galaxy.center_unit = units["galaxy"]["center"]
+    rubixdata.galaxy.halfmassrad_stars = data["subhalo_halfmassrad_stars"]
+    rubixdata.galaxy.halfmassrad_stars_unit = units["galaxy"]["halfmassrad_stars"]
+
+    for part_type in config["data"]["args"]["particle_type"]:
+        if part_type not in data["particle_data"]:
+            continue
+        holder = getattr(rubixdata, part_type)
+        for attribute, value in data["particle_data"][part_type].items():
+            setattr(holder, attribute, _like_jnp_array(value, device))
+            setattr(holder, attribute + "_unit", units[part_type][attribute])
+        logger.info(f"Centering {part_type} particles")
+        rubixdata = center_particles(rubixdata, part_type)
+        if "data" in config and "subset" in config["data"] and config["data"]["subset"]["use_subset"]:
+            size = config["data"]["subset"]["subset_size"]
+            np.random.seed(42)
+            if rubixdata.stars.coords is not None:
+                indices = np.random.choice(np.arange(len(rubixdata.stars.coords)), size=size,
+                                           replace=False)
+            elif rubixdata.gas.coords is not None:
+                indices = np.random.choice(np.arange(len(rubixdata.gas.coords)), size=size,
+                                           replace=False)
+            else:
+                raise ValueError("Neither stars nor gas coordinates are available.")
+            idx = torch.as_tensor(indices, device=device)
+            holder = getattr(rubixdata, part_type)
+            for attribute in data["particle_data"][part_type].keys():
+                v = getattr(holder, attribute)
+                # jnp indexing clamps out-of-range indices (gas subset by the
+                # star count can run past the gas arrays)
+                setattr(holder, attribute, v[idx.clamp(max=v.shape[0] - 1)])
+            logger.warning(f"The Subset value is set in config. Using only subset of size {size} "
+                           f"for {part_type}")
+    return rubixdata
+
+
+def get_rubix_data(config) -> RubixData:
+    """rubix/core/data.py:607-621 without the conversion step: the
+    reference first converts simulation snapshots (IllustrisTNG / NIHAO
+    ingest, rubix/galaxy/) into config["output_path"]/rubix_galaxy.h5 and
+    skips that when the file exists.  Ingest is not part of this package, so
+    the file must exist."""
+    if isinstance(config, str):
+        config = read_yaml(config)
+    path = os.path.join(config["output_path"], "rubix_galaxy.h5")
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} does not exist: converting simulation snapshots to the rubix format "
+            "(rubix.core.data.convert_to_rubix) is not part of rubix_amd; write the file with "
+            "rubix or any HDF5 writer in that layout")
+    return prepare_input(config)

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 from ..config import pipeline_config
 from ..pipeline import LinearTransformerPipeline
-from .data import RubixData, get_reshape_data
+from .data import RubixData, get_reshape_data, get_rubix_data
 from .ifu import (get_calculate_datacube, get_calculate_datacube_fused, get_calculate_spectra,
                   get_doppler_shift_and_resampling, get_scale_spectrum_by_mass)
 from .dust import get_calculate_dusty_datacube_fused, get_extinction
@@ -56,7 +56,13 @@ class RubixPipeline:
             fns.append(FACTORIES[step](config))
         self.pipeline = LinearTransformerPipeline(chain, fns)
 
-    def run(self, rubixdata: RubixData) -> RubixData:
+    def run(self, rubixdata: RubixData = None) -> RubixData:
+        """Runs the chain on rubixdata, or, as the reference's
+        RubixPipeline.run does, on the galaxy in
+        config["output_path"]/rubix_galaxy.h5 (rubix/core/pipeline.py:52-78,
+        138-175; get_rubix_data)."""
+        if rubixdata is None:
+            rubixdata = get_rubix_data(self.config)
         return self.pipeline.apply(rubixdata)
 
     def gradient(self, rubixdata: RubixData, loss_fn):

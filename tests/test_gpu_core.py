@@ -108,3 +108,37 @@ def test_pipeline_gradient_fused_equals_plain():
         grads.append([x.grad.cpu().numpy() for x in leaves])
     for a, b in zip(*grads):
         assert rel_err(a, b) <= TOL
+
+
+@pytest.mark.parametrize("chain", ["calc_ifu", "calc_ifu_fused"])
+def test_chain_from_rubix_galaxy_file(tmp_path, chain):
+    """RubixPipeline(config).run() with no data reads
+    config["output_path"]/rubix_galaxy.h5, as the reference's pipeline does
+    (get_rubix_data -> prepare_input: centring on the galaxy center, median
+    velocity within 10 kpc), then rotate_galaxy (face-on, the file's
+    half-mass radius) ... apply_noise.  The file is the h5py-written fixture
+    tests/golden/rubix_galaxy_small.h5 (300 stars); the oracle starts from
+    its arrays restated in numpy."""
+    import os
+    import shutil
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    shutil.copy(os.path.join(golden, "rubix_galaxy_small.h5"), tmp_path / "rubix_galaxy.h5")
+    exp = np.load(os.path.join(golden, "hdf5_expected.npz"))
+    w = S.config1(n_stars=10, seed=7)  # geometry and SSP grid only
+    cfg = S.step_config(w)
+    cfg["output_path"] = str(tmp_path)
+    out = RubixPipeline(cfg, chain).run()
+    center = exp["galaxy__center"].astype(np.float32)
+    pos, vel = exp["stars__coords"], exp["stars__velocity"]
+    vel = vel - np.median(vel[np.linalg.norm(pos - center, axis=1) < 10], axis=0)
+    pos = pos - center
+    st = {k: exp[f"stars__{k}"] for k in ("mass", "age", "metallicity")}
+    p, v = R.rotate_galaxy(pos, vel, st["mass"], 3.75, 0.0, 0.0, 0.0)
+    fields, mask = R.filter_particles(p.astype(np.float32), w.geom["edges"], st)
+    ref_stars = dict(coords=p.astype(np.float32), velocity=v.astype(np.float32), **fields)
+    ref = R.plain_forward(ref_stars, w.grid, w.geom, dtype=np.float64)
+    ref = R.apply_noise(ref, 1.0, "normal", 0, dtype=np.float64)
+    np.testing.assert_array_equal(out.stars.mask.cpu().numpy().reshape(-1), mask)
+    e = rel_err(out.stars.datacube.detach().cpu().numpy(), ref)
+    print(f"{chain} from rubix_galaxy.h5: rel64={e:.3e}, {int(mask.sum())} of {len(mask)} inside")
+    assert e <= TOL
