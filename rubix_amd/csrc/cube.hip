@@ -15,14 +15,19 @@
 //     star_prep   (thread / star, sorted order): cell, weights, a, the
 //                 in-range window [kl, kh] with the exact float predicate,
 //                 and T in O(1) from per-template prefix sums (cum).
-//     star_norm   (wavefront / (star, lambda slab)): partial Nn of the slab's
-//                 bins; star_sf adds the 8 slabs in order, sf = nan_to_num(T/Nn).
-//     cube_accum  (wavefront / (spaxel chunk, 64-bin lambda tile)): lanes own
-//                 target bins; the wavefront walks the chunk's stars in sorted
-//                 order accumulating P_j * sf in registers.  No atomics.
+//     cube_fused  (workgroup of 16 wavefronts / spaxel chunk, two per lambda
+//                 slab): per star of the chunk, P_j on every bin once from the
+//                 slab's knot window staged in LDS, Nn from the 16 shares
+//                 after a barrier, sf = nan_to_num(T/Nn), then P_j * sf into
+//                 f64 registers.  No atomics.  (Ascending targets of at most
+//                 4096 bins.)
+//     otherwise   star_norm (wavefront / (star, lambda slab)): partial Nn of
+//                 the slab's bins, star_sf adds the 8 slabs in order; then
+//                 cube_accum (wavefront / (spaxel chunk, 64-bin tile)) walks
+//                 the chunk's stars accumulating P_j * sf in registers.
 //     cube_combine: cube[p, j] = sum of the spaxel's chunk partials in chunk
 //                 order (fixed order => bitwise deterministic).
-//   The slab-mapped kernels put lambda slab b % 8 on XCD b % 8, so each
+//   The per-(star, slab) kernels put lambda slab b % 8 on XCD b % 8, so each
 //   XCD's L2 holds only its slab of the templates.
 //   Adjoint: star_bwd (wavefront / (star, lambda slab)) gathers
 //   dL/dcube[spaxel_i, :] over the slab's bins and accumulates seven sums of
@@ -87,7 +92,7 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
     const int32_t key = skey[r];
     StarState s;
     s.orig = i;
-    s.spaxel = key;
+    s.spaxel = key >= 0 ? key % st.n_spaxel : -1;  // key = band * n_spaxel + spaxel
     s.av = av ? av[i] : 0.f;
     const float m = st.mass[i];
     const float v = st.velocity[int64_t(i) * st.vel_stride + st.vel_comp];
@@ -96,7 +101,7 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
     s.inv_a = 1.0f / a;
     s.m = m;
     const Cell c = make_cell(g, st.metallicity[i], st.age[i]);
-    const bool keep = c.inside && key >= 0 && key < st.n_spaxel;
+    const bool keep = c.inside && key >= 0;
     s.row = keep ? c.row : -1;
     for (int q = 0; q < 4; ++q) s.w[q] = c.w[q];
     // d w_c / d(log Z) and d w_c / d(log t) for the adjoint's combined rows
@@ -122,6 +127,54 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
     s.sf = 0.f;
     s.Nn = 0.f;
     out[r] = s;
+  }
+}
+
+// ------------------------------------------------------------ age bands
+//
+// The fused forward walks spaxel chunks with all 8 lambda slabs of a star in
+// one workgroup, so an XCD's L2 sees every slab of the templates the stars
+// use.  The sort key band * n_spaxel + spaxel orders the stars by the age
+// band of their template cell first: chunks, and so the workgroups in
+// flight at any time, come from one band, whose template rows
+// (n_met x (n_age / bands + 1)) fit an XCD's L2.  Stars of one spaxel in
+// different bands form separate chunks; cube_combine adds a spaxel's chunks
+// band by band in a fixed order.
+constexpr int kMaxBands = 8;
+constexpr int32_t kBandKeys = 8192;          // key counts up to this get bands
+constexpr int64_t kBandBytes = int64_t(3) << 20;  // template bytes per band
+
+// Bands the workspace is laid out for: a function of the key count only.
+static int band_cap(int32_t n_keys) { return n_keys <= kBandKeys ? kMaxBands : 1; }
+
+// Bands for a template grid: about kBandBytes of templates each.
+static int band_count(const rbx_ssp& ssp, int32_t n_keys) {
+  const int64_t bytes = int64_t(ssp.n_met) * ssp.n_age * ssp.n_wave * int64_t(sizeof(float));
+  int64_t b = (bytes + kBandBytes - 1) / kBandBytes;
+  if (b > band_cap(n_keys)) b = band_cap(n_keys);
+  if (b > ssp.n_age - 1) b = ssp.n_age - 1;
+  return b < 1 ? 1 : int(b);
+}
+
+// key = band * n_spaxel + id for ids in [0, n_spaxel), -1 (sorted last,
+// dropped) otherwise; band from the star's age cell (make_cell's index).
+__global__ void band_key_kernel(SspView g, rbx_stars st, int bands, int32_t* __restrict__ keys) {
+  for (int64_t r = gtid(); r < st.n; r += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t id = st.ids[r];
+    int32_t key = -1;
+    if (id >= 0 && id < st.n_spaxel) {
+      int band = 0;
+      if (bands > 1) {
+        const float t = st.age[r];
+        const float sv = g.log_axes ? log10f(t) : t;
+        int ja = upper_bound_f(g.ax_age, g.n_age, sv);
+        ja = ja < 1 ? 1 : (ja > g.n_age - 1 ? g.n_age - 1 : ja);
+        band = ((ja - 1) * bands) / (g.n_age - 1);
+        band = band > bands - 1 ? bands - 1 : band;
+      }
+      key = band * st.n_spaxel + id;
+    }
+    keys[r] = key;
   }
 }
 
@@ -552,108 +605,153 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
   }
 }
 
-// Staged variant for ascending targets with at most kLaneBins bins per lane
-// per slab: item q = lambda slab q % 8 of W consecutive spaxel chunks (one
-// per wavefront).  Each lane keeps its bins' wavelengths and f64
-// accumulators in registers across the chunk's stars; per star the slab's
-// knot window is staged in LDS once (as in star_norm_kernel) and every bin
-// interpolates from it.  Same arithmetic per (star, bin) as
-// cube_accum_kernel and the same star order per bin, so the partials are
-// bitwise those of the per-tile kernel.
+// Bins per lane and slab of the fused pass (at most 8 x 64 per slab, so
+// targets up to 4096 bins take it).
 constexpr int kLaneBins = 8;
 
-template <int W, bool kDust>
-__global__ void __launch_bounds__(64 * W) cube_accum_staged_kernel(
-    SspView g, rbx_target tg, const StarState* __restrict__ state,
-    const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr, int slab_bins,
-    int64_t n_items, const float* __restrict__ axav, double* __restrict__ partial) {
-  __shared__ float lam_s[W][kWindow];
-  __shared__ float F_s[W][kWindow];
+// Normalisation and accumulation in one pass for ascending targets: item q
+// is spaxel chunk q, one workgroup of 16 wavefronts, two per lambda slab
+// (wavefront 2b + h takes the bins j0 + 256 h + lane + 64 t, t < 4, of slab
+// b).  For each star of the chunk the two wavefronts of a slab stage its
+// knot window in LDS (alternate 64-knot rows each) and, after a barrier,
+// evaluate P_j on their bins once (the per-bin arithmetic of
+// star_norm_kernel and cube_accum_kernel); each posts its share of
+// Nn = sum_j P_j dtarget_j; after a second barrier every wavefront adds the
+// 16 shares in a fixed order, forms sf = nan_to_num(T / Nn) and accumulates
+// P_j sf into its f64 registers.  Each (star, bin) value is computed once
+// instead of twice.  The first wavefront stores sf and Nn for the adjoint.
+constexpr int kFusedWaves = 2 * kSlabs;
+constexpr int kFusedBins = kLaneBins / 2;  // bins per lane
+
+template <bool kDust>
+__global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(8))) cube_fused_kernel(
+    SspView g, rbx_target tg, StarState* __restrict__ state, const ChunkDesc* __restrict__ desc,
+    const int32_t* __restrict__ n_chunks_ptr, int slab_bins, int64_t n_items,
+    const float* __restrict__ axav, double* __restrict__ partial) {
+  __shared__ float lam_s[kSlabs][kWindow];
+  __shared__ float F_s[kSlabs][kWindow];
+  __shared__ float nn_s[kFusedWaves];
   const int n_chunks = *n_chunks_ptr;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int slab = wave >> 1, half = wave & 1;
   const int lane = threadIdx.x & 63;
-  float* lam = lam_s[wave];
-  float* F = F_s[wave];
+  float* lam = lam_s[slab];
+  float* F = F_s[slab];
+  const uint32_t j0 = uint32_t(slab * slab_bins);
+  const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
+  const uint32_t jh = j0 + 64u * kFusedBins * half;  // this wavefront's first bin
   for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
-    const int slab = int(q & 7);
-    const int64_t chunk = (q >> 3) * W + wave;
-    if (chunk >= n_chunks) continue;
-    const uint32_t j0 = uint32_t(slab * slab_bins);
-    const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
-    if (j0 >= j1) continue;
-    const ChunkDesc d = desc[chunk];
-    float x[kLaneBins], tau[kLaneBins];
-    double acc[kLaneBins];
+    if (q >= n_chunks) break;  // chunk count is uniform: the whole workgroup leaves
+    const ChunkDesc d = desc[q];
+    float x[kFusedBins], dt[kFusedBins], tau[kFusedBins];
+    double acc[kFusedBins];
 #pragma unroll
-    for (int t = 0; t < kLaneBins; ++t) {
-      const uint32_t j = j0 + lane + 64u * t;
-      x[t] = ld_at(tg.wave, j < j1 ? j : j1 - 1);
-      tau[t] = kDust ? dust_tau(axav, j < j1 ? j : j1 - 1) : 0.f;
+    for (int t = 0; t < kFusedBins; ++t) {
+      const uint32_t j = jh + lane + 64u * t;
+      const uint32_t jc = j < j1 ? j : (j1 > j0 ? j1 - 1 : 0);
+      const Pair xd = ld_pair(tg.pairs, 2 * jc);
+      x[t] = xd.lo;
+      dt[t] = xd.hi;
+      tau[t] = kDust ? dust_tau(axav, jc) : 0.f;
       acc[t] = 0.0;
     }
     for (int r = d.begin; r < d.end; ++r) {
       const StarState s = state[r];
-      if (s.row < 0) continue;
+      if (s.row < 0) continue;  // the same star for every wavefront: uniform
+      float P[kFusedBins];
+#pragma unroll
+      for (int t = 0; t < kFusedBins; ++t) P[t] = 0.f;
       const Rows R = rows_of(g, s.row);
       const Shift h = shift_of(g, s.a, s.inv_a);
-      int k_lo, nk;
-      slab_window(g, tg, j0, j1, h, &k_lo, &nk);
-      if (nk <= kWindow) {
-        for (int k = lane; k < nk; k += 64) {
-          const uint32_t kk = uint32_t(k_lo + k);
-          lam[k] = ld_at(g.wave, kk);
-          float f = 0.f;
+      int k_lo = 0, nk = kWindow + 1;
+      if (j0 < j1) {
+        slab_window(g, tg, j0, j1, h, &k_lo, &nk);
+        if (nk <= kWindow) {
+          for (int k = 64 * half + lane; k < nk; k += 128) {
+            const uint32_t kk = uint32_t(k_lo + k);
+            lam[k] = ld_at(g.wave, kk);
+            float f = 0.f;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
-          F[k] = f;
+            for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
+            F[k] = f;
+          }
         }
-        wave_lds_sync();
-        auto bins = [&](auto interior) {
+      }
+      __syncthreads();  // both halves of every window staged
+      if (j0 < j1) {
+        if (nk <= kWindow) {
+          auto bins = [&](auto interior) {
 #pragma unroll
-          for (int t = 0; t < kLaneBins; ++t) {
-            if (j0 + lane + 64u * t < j1) {
-              const WinKnots K =
-                  win_knots<false, decltype(interior)::value>(g, x[t], h, k_lo, nk, lam);
-              const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
-              float v = (S0 + K.theta * (S1 - S0)) * s.sf;
-              if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
-              acc[t] += double(v);
+            for (int t = 0; t < kFusedBins; ++t) {
+              if (jh + lane + 64u * t < j1) {
+                const WinKnots K =
+                    win_knots<false, decltype(interior)::value>(g, x[t], h, k_lo, nk, lam);
+                const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
+                P[t] = S0 + K.theta * (S1 - S0);
+              }
+            }
+          };
+          if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) bins(std::true_type{});
+          else bins(std::false_type{});
+        } else {
+#pragma unroll
+          for (int t = 0; t < kFusedBins; ++t) {
+            if (jh + lane + 64u * t < j1) {
+              const Knots K = knots_of<false>(g, x[t], h);
+              P[t] = star_value(R, s, K);
             }
           }
-        };
-        if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) bins(std::true_type{});
-        else bins(std::false_type{});
-        wave_lds_sync();
-      } else {
+        }
+      }
+      float nn = 0.f;
 #pragma unroll
-        for (int t = 0; t < kLaneBins; ++t) {
-          if (j0 + lane + 64u * t < j1) {
-            const Knots K = knots_of<false>(g, x[t], h);
-            float v = star_value(R, s, K) * s.sf;
-            if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
-            acc[t] += double(v);
-          }
+      for (int t = 0; t < kFusedBins; ++t)
+        if (jh + lane + 64u * t < j1) nn += P[t] * dt[t];
+      nn = wave_sum(nn);
+      if (lane == 0) nn_s[wave] = nn;
+      __syncthreads();  // every share posted; every window read
+      float Nn = 0.f;
+#pragma unroll
+      for (int b = 0; b < kFusedWaves; ++b) Nn += nn_s[b];
+      float sf = s.T / Nn;
+      if (isnan(sf)) sf = 0.f;
+      else if (isinf(sf)) sf = sf > 0.f ? FLT_MAX : -FLT_MAX;
+      if (wave == 0 && lane == 0) {
+        state[r].sf = sf;
+        state[r].Nn = Nn;
+      }
+#pragma unroll
+      for (int t = 0; t < kFusedBins; ++t) {
+        if (jh + lane + 64u * t < j1) {
+          float v = P[t] * sf;
+          if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
+          acc[t] += double(v);
         }
       }
     }
 #pragma unroll
-    for (int t = 0; t < kLaneBins; ++t) {
-      const uint32_t j = j0 + lane + 64u * t;
-      if (j < j1) partial[chunk * tg.n + j] = acc[t];
+    for (int t = 0; t < kFusedBins; ++t) {
+      const uint32_t j = jh + lane + 64u * t;
+      if (j < j1) partial[q * tg.n + j] = acc[t];
     }
   }
 }
 
+// chunk_off is indexed by key band * n_spaxel + p: the chunks of spaxel p
+// are added band by band, each band's in chunk order.
 __global__ void cube_combine_kernel(const double* __restrict__ partial,
                                     const int32_t* __restrict__ chunk_off, int32_t n_spaxel,
-                                    int32_t nt, float* __restrict__ cube) {
+                                    int32_t bands, int32_t nt, float* __restrict__ cube) {
   const int64_t total = int64_t(n_spaxel) * nt;
   for (int64_t idx = gtid(); idx < total; idx += int64_t(gridDim.x) * blockDim.x) {
     const int32_t p = int32_t(idx / nt);
     const int32_t j = int32_t(idx - int64_t(p) * nt);
-    const int32_t c0 = chunk_off[p], c1 = chunk_off[p + 1];
     double s = 0.0;
-    for (int32_t c = c0; c < c1; ++c) s += partial[int64_t(c) * nt + j];
+    for (int32_t b = 0; b < bands; ++b) {
+      const int64_t k = int64_t(b) * n_spaxel + p;
+      const int32_t c0 = chunk_off[k], c1 = chunk_off[k + 1];
+      for (int32_t c = c0; c < c1; ++c) s += partial[int64_t(c) * nt + j];
+    }
     cube[idx] = float(s);
   }
 }
@@ -1096,7 +1194,7 @@ __global__ void ssp_locate_kernel(const float* __restrict__ wave, int nw, float 
 // state and the slab partials) come first and do not depend on the chunk
 // size, so the adjoint may run with any tuning.
 struct CubeLayout {
-  int32_t *perm, *skey, *offsets, *chunk_off, *scan_blocks;
+  int32_t *perm, *skey, *offsets, *chunk_off, *scan_blocks, *keys;
   StarState* state;
   ChunkDesc* desc;
   double* partial;
@@ -1113,8 +1211,10 @@ struct CubeLayout {
 template <class C>
 static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, CubeLayout* L,
                         bool per_star_only = false) {
-  const int64_t max_chunks = (n + chunk - 1) / chunk + n_keys;
-  const size_t sort_bytes = sort_workspace_size(n, n_keys);
+  // sort keys band * n_spaxel + spaxel: up to band_cap(n_keys) * n_keys
+  const int64_t nk = int64_t(band_cap(n_keys)) * n_keys;
+  const int64_t max_chunks = (n + chunk - 1) / chunk + nk;
+  const size_t sort_bytes = sort_workspace_bound(n, int32_t(nk));
   const size_t n1 = size_t(n > 0 ? n : 1);
   auto* perm = c.template take<int32_t>(n1);
   auto* skey = c.template take<int32_t>(n1);
@@ -1135,9 +1235,10 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, 
     }
     return;
   }
-  auto* offs = c.template take<int32_t>(size_t(n_keys) + 1);
-  auto* coff = c.template take<int32_t>(size_t(n_keys) + 1);
-  auto* blocks = c.template take<int32_t>((size_t(n_keys) + 1) / 4096 + 2);
+  auto* offs = c.template take<int32_t>(size_t(nk) + 1);
+  auto* coff = c.template take<int32_t>(size_t(nk) + 1);
+  auto* blocks = c.template take<int32_t>((size_t(nk) + 1) / 4096 + 2);
+  auto* keys = c.template take<int32_t>(n1);
   auto* desc = c.template take<ChunkDesc>(size_t(max_chunks > 0 ? max_chunks : 1));
   auto* part = c.template take<double>(size_t(max_chunks > 0 ? max_chunks : 1) * nt);
   auto* sws = c.template take<char>(sort_bytes);
@@ -1152,6 +1253,7 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, 
     L->offsets = reinterpret_cast<int32_t*>(offs);
     L->chunk_off = reinterpret_cast<int32_t*>(coff);
     L->scan_blocks = reinterpret_cast<int32_t*>(blocks);
+    L->keys = reinterpret_cast<int32_t*>(keys);
     L->desc = reinterpret_cast<ChunkDesc*>(desc);
     L->partial = reinterpret_cast<double*>(part);
     L->sort_ws = sws;
@@ -1293,9 +1395,9 @@ int build_chunks(const int32_t* offsets, int32_t ns, int chunk, int32_t* chunk_o
 }
 
 void combine_chunks(const double* partial, const int32_t* chunk_off, int32_t ns, int32_t nt,
-                    float* cube, hipStream_t s) {
+                    float* cube, hipStream_t s, int32_t bands) {
   hipLaunchKernelGGL(cube_combine_kernel, dim3(grid_for(int64_t(ns) * nt, 256)), dim3(256), 0, s,
-                     partial, chunk_off, ns, nt, cube);
+                     partial, chunk_off, ns, bands, nt, cube);
 }
 
 size_t cube_workspace_impl(int64_t n, int32_t n_keys, int32_t nt, const Tune& t) {
@@ -1321,14 +1423,26 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   cube_layout(c, n, stars->n_keys, nt, t.chunk, &L);
   if (!c.ok) return failf(RBX_EWORKSPACE, "%s: workspace too small", who);
   const SspView g = view_of(*ssp);
-  rc = spaxel_sort_impl(stars->ids, n, stars->n_keys, L.perm, L.skey, L.offsets, L.sort_ws,
+  const int32_t ns = stars->n_spaxel;
+  const int bands = band_count(*ssp, stars->n_keys);
+  const int32_t n_band_keys = bands * ns;  // <= band_cap(n_keys) * n_keys
+  if (n > 0)
+    hipLaunchKernelGGL(band_key_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *stars, bands,
+                       L.keys);
+  rc = spaxel_sort_impl(L.keys, n, n_band_keys, L.perm, L.skey, L.offsets, L.sort_ws,
                         L.sort_ws_bytes, s);
   if (rc) return rc;
   const int n_tiles = (nt + kTile - 1) / kTile;
   const int tps = (n_tiles + kSlabs - 1) / kSlabs;
+  // ascending targets with at most kLaneBins bins per lane and slab: one
+  // fused pass (cube_fused_kernel); otherwise star_norm + star_sf, then the
+  // per-tile accumulation
+  const bool fused = target->ascending && tps <= kLaneBins;
   if (n > 0) {
     hipLaunchKernelGGL(star_prep_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *target,
                        *stars, L.perm, L.skey, ext ? ext->av : nullptr, L.state, L.gradw);
+  }
+  if (n > 0 && !fused) {
     const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
     const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
     switch (t.waves) {
@@ -1341,20 +1455,19 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
     hipLaunchKernelGGL(star_sf_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L.nn_part,
                        L.state);
   }
-  const int32_t ns = stars->n_spaxel;
-  rc = build_chunks(L.offsets, ns, t.chunk, L.chunk_off, L.scan_blocks, L.desc, s);
+  rc = build_chunks(L.offsets, n_band_keys, t.chunk, L.chunk_off, L.scan_blocks, L.desc, s);
   if (rc) return rc;
   const int tpx = (n_tiles + 7) / 8;
-  if (n > 0 && target->ascending && tpx <= kLaneBins) {
-    const int64_t items = int64_t(8) * ((L.max_chunks + 3) / 4);
-    const int64_t grid = item_grid(items, t.persist, 256);
+  if (n > 0 && fused) {
+    const int64_t items = L.max_chunks;
+    const int64_t grid = item_grid(items, t.persist, 64 * kFusedWaves);
     if (axav)
-      hipLaunchKernelGGL((cube_accum_staged_kernel<4, true>), dim3(grid), dim3(256), 0, s, g,
-                         *target, L.state, L.desc, L.chunk_off + ns, tpx * kTile, items, axav,
+      hipLaunchKernelGGL(cube_fused_kernel<true>, dim3(grid), dim3(64 * kFusedWaves), 0, s, g,
+                         *target, L.state, L.desc, L.chunk_off + n_band_keys, tps * kTile, items, axav,
                          L.partial);
     else
-      hipLaunchKernelGGL((cube_accum_staged_kernel<4, false>), dim3(grid), dim3(256), 0, s, g,
-                         *target, L.state, L.desc, L.chunk_off + ns, tpx * kTile, items, nullptr,
+      hipLaunchKernelGGL(cube_fused_kernel<false>, dim3(grid), dim3(64 * kFusedWaves), 0, s, g,
+                         *target, L.state, L.desc, L.chunk_off + n_band_keys, tps * kTile, items, nullptr,
                          L.partial);
     if ((rc = check_launch(who))) return rc;
   } else if (n > 0) {
@@ -1364,14 +1477,14 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
     const int64_t grid = item_grid(items, t.persist, 64);
     if (axav)
       hipLaunchKernelGGL(cube_accum_kernel<true>, dim3(grid), dim3(64), 0, s, g, *target, L.state,
-                         L.desc, L.chunk_off + ns, n_tiles, tpx, items, axav, L.partial);
+                         L.desc, L.chunk_off + n_band_keys, n_tiles, tpx, items, axav, L.partial);
     else
       hipLaunchKernelGGL(cube_accum_kernel<false>, dim3(grid), dim3(64), 0, s, g, *target,
-                         L.state, L.desc, L.chunk_off + ns, n_tiles, tpx, items, nullptr,
+                         L.state, L.desc, L.chunk_off + n_band_keys, n_tiles, tpx, items, nullptr,
                          L.partial);
     if ((rc = check_launch(who))) return rc;
   }
-  combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s);
+  combine_chunks(L.partial, L.chunk_off, ns, nt, cube, s, bands);
   return check_launch(who);
 }
 

@@ -280,6 +280,24 @@ size_t sort_workspace_size(int64_t n, int32_t n_keys) {
   return a.s.used;
 }
 
+// Workspace enough for any key count in [1, max_keys].  The count table is
+// (keys + 1) x tiles; tiles of 4096 stars double while the table would pass
+// 2^24 entries (up to tiles of 2^24 stars).  Below the first doubling the
+// table grows with the key count; past it, it stays under 2^24 entries
+// until the tile size is capped, and then grows again with the key count.
+// So max(table at max_keys, 2^24 entries) bounds every key count.
+size_t sort_workspace_bound(int64_t n, int32_t max_keys) {
+  SortPlan p = make_plan(n, max_keys);
+  const int64_t tiles = n > 0 ? (n + 4095) / 4096 : 1;
+  if ((int64_t(max_keys) + 1) * tiles > (int64_t(1) << 24) && p.table < (int64_t(1) << 24)) {
+    p.table = int64_t(1) << 24;
+    p.nblk = (p.table + kScanBlock - 1) / kScanBlock;
+  }
+  SizerAdapter a;
+  sort_layout(a, p, nullptr, nullptr);
+  return a.s.used;
+}
+
 // Sort used by the fused path too; the workspace is carved by the caller.
 int spaxel_sort_impl(const int32_t* ids, int64_t n, int32_t n_keys, int32_t* perm,
                      int32_t* sorted, int32_t* offsets, void* ws, size_t ws_bytes,

@@ -18,13 +18,16 @@ struct ChunkDesc;
 // number of chunks.
 int build_chunks(const int32_t* offsets, int32_t ns, int chunk, int32_t* chunk_off,
                  int32_t* scan_blocks, ChunkDesc* desc, hipStream_t s);
-// cube[p, j] = sum of partial[c, j] over the chunks of spaxel p, in order.
+// cube[p, j] = sum of partial[c, j] over the chunks of spaxel p, in order;
+// with bands > 1 the chunk offsets are indexed by band * ns + p and each
+// spaxel's bands are added in band order.
 void combine_chunks(const double* partial, const int32_t* chunk_off, int32_t ns, int32_t nt,
-                    float* cube, hipStream_t s);
+                    float* cube, hipStream_t s, int32_t bands = 1);
 int spaxel_sort_impl(const int32_t* ids, int64_t n, int32_t n_keys, int32_t* perm,
                      int32_t* sorted, int32_t* offsets, void* ws, size_t ws_bytes,
                      hipStream_t s);
 size_t sort_workspace_size(int64_t n, int32_t n_keys);
+size_t sort_workspace_bound(int64_t n, int32_t max_keys);  // any key count <= max_keys
 int exclusive_scan_i32(int32_t* data, int64_t n, int32_t* blocks, hipStream_t s);
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 
