@@ -5,8 +5,11 @@
 // per-star kernels and the chunked cube reduction run once over the whole
 // batch and write every galaxy's cube rows of one batch cube.  The per-star
 // work does not depend on the galaxy, and chunks never cross a spaxel, so a
-// galaxy's cube and gradients are bitwise those of a single-galaxy run with
-// the same tuning.
+// galaxy's gradients are bitwise those of a single-galaxy run with the same
+// tuning, and so is its cube when both runs use the same number of age
+// bands (a batch's key count gives it one band; a single galaxy gets more
+// only for template grids above one band's budget, and then its cube
+// differs by the order of the f64 chunk additions alone).
 #include "cube_impl.h"
 #include "rbx_common.h"
 
