@@ -7,7 +7,9 @@ header holds the run's configuration keys, and an IMAGE extension "DATA"
 holding datacube.T (NAXIS1 = the cube's first axis, NAXIS3 = wavelength) as
 IEEE float32 with the cube's world coordinate keys.  load_fits returns
 (primary header, data header, cube in the (nb, nb, n_wave) layout); the
-reference returns an mpdaf Cube, which is not available here."""
+reference returns an mpdaf Cube, which is not available here.  The bytes
+are those astropy.io.fits writes for the reference's HDU list (checked
+against an astropy-written file in tests/test_fits.py)."""
 from __future__ import annotations
 
 import os
@@ -22,6 +24,24 @@ BLOCK = 2880
 CARD = 80
 
 
+def _float(v: float) -> str:
+    """A real value as astropy.io.fits (which the reference writes with)
+    renders it in a fixed-format card: 16 significant digits ('G'), a '.0'
+    appended to integral values, a two-digit signed exponent, and a mantissa
+    cut short (not rounded) so that the field fits its 20 columns."""
+    s = f"{float(v):.16G}"
+    if "E" in s:
+        mant, exp = s.split("E")
+        sign = exp[0] if exp[0] in "+-" else ""
+        s = f"{mant}E{sign}{int(exp.lstrip('+-')):02d}"
+    elif "." not in s and "INF" not in s and "NAN" not in s:
+        s += ".0"
+    if len(s) > 20:
+        e = s.find("E")
+        s = s[:20] if e < 0 else s[:20 - (len(s) - e)] + s[e:]
+    return s
+
+
 def _value(v) -> str:
     """Fixed-format value field (columns 11-80 of a card)."""
     if isinstance(v, (bool, np.bool_)):
@@ -29,10 +49,7 @@ def _value(v) -> str:
     if isinstance(v, (int, np.integer)):
         return f"{int(v):>20}"
     if isinstance(v, (float, np.floating)):
-        s = repr(float(v)).upper()
-        if "E" not in s and "." not in s and "INF" not in s and "NAN" not in s:
-            s += ".0"
-        return f"{s:>20}"
+        return f"{_float(v):>20}"
     quoted = "'" + f"{str(v).replace(chr(39), chr(39) * 2):<8}" + "'"
     return f"{quoted:<20}"
 
@@ -62,11 +79,14 @@ def write_cube_fits(path: str, cube: np.ndarray, primary: list, data_keys: list)
     """Primary HDU (no data) + IMAGE extension of cube.T as float32."""
     arr = np.ascontiguousarray(np.asarray(cube, dtype=np.float32).T)
     shape_fits = arr.shape[::-1]  # NAXIS1 is the fastest (last numpy) axis
-    prim = [("SIMPLE", True, "conforms to FITS standard"), ("BITPIX", 8), ("NAXIS", 0),
-            ("EXTEND", True)] + list(primary)
-    ext = [("XTENSION", "IMAGE", "Image extension"), ("BITPIX", -32), ("NAXIS", arr.ndim)]
+    # mandatory cards with the comments astropy.io.fits gives them
+    prim = [("SIMPLE", True, "conforms to FITS standard"), ("BITPIX", 8, "array data type"),
+            ("NAXIS", 0, "number of array dimensions"), ("EXTEND", True)] + list(primary)
+    ext = [("XTENSION", "IMAGE", "Image extension"), ("BITPIX", -32, "array data type"),
+           ("NAXIS", arr.ndim, "number of array dimensions")]
     ext += [(f"NAXIS{i + 1}", int(n)) for i, n in enumerate(shape_fits)]
-    ext += [("PCOUNT", 0), ("GCOUNT", 1)] + list(data_keys)
+    ext += [("PCOUNT", 0, "number of parameters"), ("GCOUNT", 1, "number of groups")]
+    ext += list(data_keys)
     data = arr.astype(">f4").tobytes()
     data += b"\0" * ((-len(data)) % BLOCK)
     with open(path, "wb") as f:

@@ -1,6 +1,10 @@
 """CPU: the FITS writer / reader (rubix_amd/core/fits.py) against the FITS
-Standard 4.0 layout, byte for byte where the standard fixes it, and a round
-trip of a datacube through store_fits / load_fits."""
+Standard 4.0 layout, byte for byte against a file astropy.io.fits wrote for
+the reference's HDU list (tests/golden/reference_cube.fits, made by
+tests/golden/make_fits_fixture.py), and a round trip of a datacube through
+store_fits / load_fits."""
+import os
+
 import numpy as np
 import torch
 
@@ -17,12 +21,12 @@ def test_layout_follows_the_standard(tmp_path):
     assert len(raw) % 2880 == 0
     # mandatory first cards, fixed format: logical / integer values end in column 30
     assert raw[:80].decode() == "SIMPLE  =                    T / conforms to FITS standard".ljust(80)
-    assert raw[80:160].decode() == "BITPIX  =                    8".ljust(80)
-    assert raw[160:240].decode() == "NAXIS   =                    0".ljust(80)
+    assert raw[80:160].decode() == "BITPIX  =                    8 / array data type".ljust(80)
+    assert raw[160:240].decode() == "NAXIS   =                    0 / number of array dimensions".ljust(80)
     ext = raw.index(b"XTENSION")
     assert ext == 2880
     assert raw[ext:ext + 80].decode() == "XTENSION= 'IMAGE   '           / Image extension".ljust(80)
-    assert raw[ext + 80:ext + 160].decode() == "BITPIX  =                  -32".ljust(80)
+    assert raw[ext + 80:ext + 160].decode() == "BITPIX  =                  -32 / array data type".ljust(80)
     assert raw[ext + 240:ext + 320].decode() == "NAXIS1  =                    2".ljust(80)  # cube.T
     assert raw[ext + 400:ext + 480].decode() == "NAXIS3  =                    4".ljust(80)
     # data: the second header is one block; big-endian cube.T follows
@@ -48,3 +52,43 @@ def test_store_and_load_round_trip(tmp_path):
     assert prim["N_DISTR"] == "normal" and prim["DIST_Z"] == cfg["galaxy"]["dist_z"]
     assert hdr["EXTNAME"] == "DATA" and hdr["CTYPE3"] == "AWAV" and hdr["CRPIX1"] == 2.0
     assert hdr["NAXIS1"] == 5 and hdr["NAXIS3"] == 7
+
+
+def _reference_config():
+    """The run keys tests/golden/make_fits_fixture.py wrote with astropy."""
+    return {
+        "device": "cpu",
+        "pipeline": {"name": "calc_ifu"},
+        "galaxy": {"dist_z": 0.1, "rotation": {"type": "face-on"}},
+        "simulation": {"name": "IllustrisTNG"},
+        "data": {"args": {"snapshot": 99}, "load_galaxy_args": {"id": 12},
+                 "subset": {"use_subset": True}},
+        "ssp": {"template": {"name": "BruzualCharlot2003"}},
+        "telescope": {"name": "MUSE", "psf": {"name": "gaussian", "size": 5, "sigma": 0.6},
+                      "lsf": {"sigma": 0.5},
+                      "noise": {"signal_to_noise": 100, "noise_distribution": "normal"}},
+        "cosmology": {"name": "PLANCK15"},
+    }
+
+
+def test_bytes_equal_astropy_output(tmp_path):
+    """store_fits writes exactly the bytes astropy.io.fits writes for the
+    reference's store_fits (rubix/core/fits.py:13-100) on the same run keys
+    and cube: card order, value formatting (16-digit reals cut to 20
+    columns, padded strings), the mandatory cards' comments, big-endian
+    datacube.T and the block padding."""
+    from rubix_amd.core.fits import read_cube_fits
+    from rubix_amd.core.telescope import get_telescope
+    golden = os.path.join(os.path.dirname(__file__), "golden", "reference_cube.fits")
+    tel = get_telescope({"telescope": {"name": "MUSE"}})
+    assert (tel.spatial_res, tel.wave_res, tel.wave_range[0]) == (0.2, 1.25, 4700.15)
+    _, _, cube = read_cube_fits(golden)
+    rd = RubixData(stars=StarsData(datacube=torch.from_numpy(cube)))
+    name = store_fits(_reference_config(), rd, str(tmp_path) + "/")
+    ours = open(name, "rb").read()
+    ref = open(golden, "rb").read()
+    if ours != ref:  # name the first differing card
+        i = next(k for k in range(min(len(ours), len(ref))) if ours[k] != ref[k])
+        c = i // 80 * 80
+        raise AssertionError(f"byte {i}: {ours[c:c + 80]!r} != {ref[c:c + 80]!r}")
+    assert len(ours) == len(ref) == 3 * 2880
