@@ -17,10 +17,10 @@
 //                 and T in O(1) from per-template prefix sums (cum).
 //     cube_fused  (workgroup of 16 wavefronts / spaxel chunk, two per lambda
 //                 slab): per star of the chunk, P_j on every bin once from the
-//                 slab's knot window staged in LDS, Nn from the 16 shares
-//                 after a barrier, sf = nan_to_num(T/Nn), then P_j * sf into
-//                 f64 registers.  No atomics.  (Ascending targets of at most
-//                 4096 bins.)
+//                 knot window of each wavefront's 256 bins staged in its LDS,
+//                 Nn from the 16 shares after a barrier, sf = nan_to_num(T/Nn),
+//                 then P_j * sf into f64 registers.  No atomics.  (Ascending
+//                 targets of at most 4096 bins.)
 //     otherwise   star_norm (wavefront / (star, lambda slab)): partial Nn of
 //                 the slab's bins, star_sf adds the 8 slabs in order; then
 //                 cube_accum (wavefront / (spaxel chunk, 64-bin tile)) walks
@@ -612,34 +612,40 @@ constexpr int kLaneBins = 8;
 // Normalisation and accumulation in one pass for ascending targets: item q
 // is spaxel chunk q, one workgroup of 16 wavefronts, two per lambda slab
 // (wavefront 2b + h takes the bins j0 + 256 h + lane + 64 t, t < 4, of slab
-// b).  For each star of the chunk the two wavefronts of a slab stage its
-// knot window in LDS (alternate 64-knot rows each) and, after a barrier,
-// evaluate P_j on their bins once (the per-bin arithmetic of
-// star_norm_kernel and cube_accum_kernel); each posts its share of
-// Nn = sum_j P_j dtarget_j; after a second barrier every wavefront adds the
-// 16 shares in a fixed order, forms sf = nan_to_num(T / Nn) and accumulates
-// P_j sf into its f64 registers.  Each (star, bin) value is computed once
-// instead of twice.  The first wavefront stores sf and Nn for the adjoint.
+// b).  Per star of the chunk each wavefront stages the knot window of its
+// own 256 bins in its LDS (no other wavefront reads it) and evaluates P_j on
+// them once (the per-bin arithmetic of star_norm_kernel and
+// cube_accum_kernel), then posts its share of Nn = sum_j P_j dtarget_j;
+// after one barrier every wavefront adds the 16 shares in a fixed order,
+// forms sf = nan_to_num(T / Nn) and accumulates P_j sf into its f64
+// registers.  The shares are double-buffered, so a wavefront may post the
+// next star's share while another still reads this one's.  Each (star, bin)
+// value is computed once instead of twice.  The first wavefront stores sf
+// and Nn for the adjoint.
 constexpr int kFusedWaves = 2 * kSlabs;
 constexpr int kFusedBins = kLaneBins / 2;  // bins per lane
+constexpr int kHalfWindow = 320;           // knots per staged half-slab window
 
 template <bool kDust>
-__global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(8))) cube_fused_kernel(
-    SspView g, rbx_target tg, StarState* __restrict__ state, const ChunkDesc* __restrict__ desc,
-    const int32_t* __restrict__ n_chunks_ptr, int slab_bins, int64_t n_items,
-    const float* __restrict__ axav, double* __restrict__ partial) {
-  __shared__ float lam_s[kSlabs][kWindow];
-  __shared__ float F_s[kSlabs][kWindow];
-  __shared__ float nn_s[kFusedWaves];
+__global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(8)))
+cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
+                  const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr,
+                  int slab_bins, int64_t n_items, const float* __restrict__ axav,
+                  double* __restrict__ partial) {
+  __shared__ float lam_s[kFusedWaves][kHalfWindow];
+  __shared__ float F_s[kFusedWaves][kHalfWindow];
+  __shared__ float nn_s[2][kFusedWaves];
   const int n_chunks = *n_chunks_ptr;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int slab = wave >> 1, half = wave & 1;
   const int lane = threadIdx.x & 63;
-  float* lam = lam_s[slab];
-  float* F = F_s[slab];
+  float* lam = lam_s[wave];
+  float* F = F_s[wave];
   const uint32_t j0 = uint32_t(slab * slab_bins);
   const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
-  const uint32_t jh = j0 + 64u * kFusedBins * half;  // this wavefront's first bin
+  // this wavefront's bins [jh, jh_end)
+  const uint32_t jh = j0 + 64u * kFusedBins * half;
+  const uint32_t jh_end = jh + 64u * kFusedBins < j1 ? jh + 64u * kFusedBins : j1;
   for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
     if (q >= n_chunks) break;  // chunk count is uniform: the whole workgroup leaves
     const ChunkDesc d = desc[q];
@@ -655,19 +661,20 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
       tau[t] = kDust ? dust_tau(axav, jc) : 0.f;
       acc[t] = 0.0;
     }
+    int par = 0;
     for (int r = d.begin; r < d.end; ++r) {
       const StarState s = state[r];
       if (s.row < 0) continue;  // the same star for every wavefront: uniform
       float P[kFusedBins];
 #pragma unroll
       for (int t = 0; t < kFusedBins; ++t) P[t] = 0.f;
-      const Rows R = rows_of(g, s.row);
-      const Shift h = shift_of(g, s.a, s.inv_a);
-      int k_lo = 0, nk = kWindow + 1;
-      if (j0 < j1) {
-        slab_window(g, tg, j0, j1, h, &k_lo, &nk);
-        if (nk <= kWindow) {
-          for (int k = 64 * half + lane; k < nk; k += 128) {
+      if (jh < jh_end) {
+        const Rows R = rows_of(g, s.row);
+        const Shift h = shift_of(g, s.a, s.inv_a);
+        int k_lo, nk;
+        slab_window(g, tg, jh, jh_end, h, &k_lo, &nk);
+        if (nk <= kHalfWindow) {
+          for (int k = lane; k < nk; k += 64) {
             const uint32_t kk = uint32_t(k_lo + k);
             lam[k] = ld_at(g.wave, kk);
             float f = 0.f;
@@ -675,15 +682,11 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
             for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
             F[k] = f;
           }
-        }
-      }
-      __syncthreads();  // both halves of every window staged
-      if (j0 < j1) {
-        if (nk <= kWindow) {
+          wave_lds_sync();
           auto bins = [&](auto interior) {
 #pragma unroll
             for (int t = 0; t < kFusedBins; ++t) {
-              if (jh + lane + 64u * t < j1) {
+              if (jh + lane + 64u * t < jh_end) {
                 const WinKnots K =
                     win_knots<false, decltype(interior)::value>(g, x[t], h, k_lo, nk, lam);
                 const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
@@ -693,10 +696,11 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
           };
           if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) bins(std::true_type{});
           else bins(std::false_type{});
+          wave_lds_sync();  // this window's reads done before the next star's stores
         } else {
 #pragma unroll
           for (int t = 0; t < kFusedBins; ++t) {
-            if (jh + lane + 64u * t < j1) {
+            if (jh + lane + 64u * t < jh_end) {
               const Knots K = knots_of<false>(g, x[t], h);
               P[t] = star_value(R, s, K);
             }
@@ -706,13 +710,14 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
       float nn = 0.f;
 #pragma unroll
       for (int t = 0; t < kFusedBins; ++t)
-        if (jh + lane + 64u * t < j1) nn += P[t] * dt[t];
+        if (jh + lane + 64u * t < jh_end) nn += P[t] * dt[t];
       nn = wave_sum(nn);
-      if (lane == 0) nn_s[wave] = nn;
-      __syncthreads();  // every share posted; every window read
+      if (lane == 0) nn_s[par][wave] = nn;
+      __syncthreads();  // every share of this star posted
       float Nn = 0.f;
 #pragma unroll
-      for (int b = 0; b < kFusedWaves; ++b) Nn += nn_s[b];
+      for (int b = 0; b < kFusedWaves; ++b) Nn += nn_s[par][b];
+      par ^= 1;
       float sf = s.T / Nn;
       if (isnan(sf)) sf = 0.f;
       else if (isinf(sf)) sf = sf > 0.f ? FLT_MAX : -FLT_MAX;
@@ -722,7 +727,7 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
       }
 #pragma unroll
       for (int t = 0; t < kFusedBins; ++t) {
-        if (jh + lane + 64u * t < j1) {
+        if (jh + lane + 64u * t < jh_end) {
           float v = P[t] * sf;
           if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
           acc[t] += double(v);
@@ -732,7 +737,7 @@ __global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_
 #pragma unroll
     for (int t = 0; t < kFusedBins; ++t) {
       const uint32_t j = jh + lane + 64u * t;
-      if (j < j1) partial[q * tg.n + j] = acc[t];
+      if (j < jh_end) partial[q * tg.n + j] = acc[t];
     }
   }
 }
