@@ -9,7 +9,8 @@
 // tuning, and so is its cube when both runs use the same number of age
 // bands (a batch's key count gives it one band; a single galaxy gets more
 // only for template grids above one band's budget, and then its cube
-// differs by the order of the f64 chunk additions alone).
+// differs only by the order of the f64 chunk additions, which add float32
+// terms exactly while a spaxel's terms span less than about 2^29).
 #include "cube_impl.h"
 #include "rbx_common.h"
 

@@ -194,3 +194,31 @@ def test_dense_template_grid_uses_gather_path():
     e = rel_err(cube.cpu().numpy().reshape(ref.shape), ref)
     print(f"dense grid cube rel64={e:.3e}")
     assert e <= TOL
+
+
+def test_age_bands_change_only_the_summation_order():
+    """cfg2's 24 MB template grid sorts the stars by (age band, spaxel) and
+    reduces each spaxel's chunks band by band when the key count allows
+    (n_keys <= 8192); with n_keys = 8193 the same stars take one band.
+    The per-star work is the same, so the gradients are bitwise equal, and
+    the cubes can differ only by the order of the f64 chunk additions.  The
+    terms are float32 values, whose f64 sums are exact while they span less
+    than about 2^29 in magnitude, so at 100 000 stars (~40 per spaxel) the
+    cubes agree bit for bit as well (the check allows rounding)."""
+    w = cfg2_sample(100_000)
+    t, ssp, tgt, edges = _device_inputs(w)
+    ids = ops.spaxel_assign(t["coords"], edges)
+    nb = w.geom["nb"]
+    out = {}
+    for n_keys in (nb * nb, 8193):
+        cube, ctx = ops.cube_forward(ssp, tgt, t["mass"], t["age"], t["metallicity"],
+                                     t["velocity"], ids, nb * nb, n_keys)
+        gen = torch.Generator(device=DEV).manual_seed(3)
+        dc = torch.randn(cube.shape, generator=gen, device=DEV)
+        out[n_keys] = (cube.cpu().numpy(), [g.cpu().numpy() for g in ops.cube_backward(ssp, tgt, ctx, dc)])
+    (c8, g8), (c1, g1) = out[nb * nb], out[8193]
+    e = rel_err(c8, c1)
+    print(f"8 bands vs 1 band: cube rel={e:.3e}, max|diff|/max={max_rel(c8, c1):.3e}")
+    assert e <= 1e-7 and max_rel(c8, c1) <= 1e-6
+    for a, b in zip(g8, g1):
+        np.testing.assert_array_equal(a, b)
