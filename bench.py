@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # star_grad_kernel) at 1M stars -> 50x50x3721 with the 12x100x5000 grid:
 # rocprofv3 FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), the factors from a 1 GiB
 # device copy (bench.py --calibrate-copy); profiles/hbm_traffic_pmc_1M_50x50x3721.md.
-TRAFFIC_BWD_1M_BYTES = 3.603e9
+TRAFFIC_BWD_1M_BYTES = 3.602e9
 
 
 def log(*a):
