@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(64 * W) star_norm_kernel(SspView g, rbx_target
           float f = 0.f;
 #pragma unroll
           for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
-          F[k] = f;
+          F[k] = f * s.m;  // S_k = F_k m, rounded once per knot as the reference's spectra
         }
         wave_lds_sync();
         auto bins = [&](auto interior) {
@@ -488,7 +488,7 @@ __global__ void __launch_bounds__(64 * W) star_norm_kernel(SspView g, rbx_target
           for (uint32_t j = j0 + t.lane; j < j1; j += 64) {
             const Pair xd = ld_pair(tg.pairs, 2 * j);
             const WinKnots K = win_knots<false, decltype(interior)::value>(g, xd.lo, h, k_lo, nk, lam);
-            const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
+            const float S0 = F[K.li], S1 = F[K.li + 1];
             acc += (S0 + K.theta * (S1 - S0)) * xd.hi;
           }
         };
@@ -680,7 +680,7 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
             float f = 0.f;
 #pragma unroll
             for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
-            F[k] = f;
+            F[k] = f * s.m;  // S_k = F_k m, rounded once per knot as the reference's spectra
           }
           wave_lds_sync();
           auto bins = [&](auto interior) {
@@ -689,7 +689,7 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
               if (jh + lane + 64u * t < jh_end) {
                 const WinKnots K =
                     win_knots<false, decltype(interior)::value>(g, x[t], h, k_lo, nk, lam);
-                const float S0 = F[K.li] * s.m, S1 = F[K.li + 1] * s.m;
+                const float S0 = F[K.li], S1 = F[K.li + 1];
                 P[t] = S0 + K.theta * (S1 - S0);
               }
             }
