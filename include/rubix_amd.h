@@ -143,6 +143,10 @@ typedef struct rbx_stars {
  * deterministic path.  Writes cube (n_spaxel, target.n).  The workspace
  * keeps the per-star state that rbx_cube_bwd reads: keep it untouched
  * between the two calls.  Per-star spectra are never materialised.
+ * For n_keys <= 8192 the workspace also holds the chunk partials of up to 8
+ * template-age bands per spaxel (the stars are reduced band by band when the
+ * template grid exceeds one band's ~3 MB, which keeps each XCD's L2 on one
+ * band's templates), so it grows with 8 * n_keys rows of n_target doubles.
  */
 size_t rbx_cube_workspace_size(int64_t n, int32_t n_keys, int32_t n_target);
 int rbx_cube_fwd(const rbx_ssp* ssp, const rbx_target* target,
