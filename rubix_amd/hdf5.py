@@ -236,7 +236,7 @@ class H5File:
         while b.pos + hdr <= end:
             mtype = b.u(1)
             msize = b.u(2)
-            b.skip(1)  # message flags
+            mflags = b.u(1)
             if flags & 0x04:
                 b.skip(2)  # creation order
             if b.pos + msize > end:
@@ -246,7 +246,7 @@ class H5File:
                 cb = _Buf(raw)
                 conts.append((cb.u(self.so), cb.u(self.sl)))
             if mtype != 0:
-                out.append((mtype, 0, raw))
+                out.append((mtype, mflags, raw))
         for addr, length in conts:
             if self.data[addr:addr + 4] != b"OCHK":
                 raise HDF5Error("bad object header continuation block")
@@ -406,7 +406,12 @@ class Dataset(_Object):
 
     @property
     def _type(self) -> _Type:
-        return _parse_type(_Buf(self._msg(0x0003)[0]))
+        for t, flags, raw in self._msgs:
+            if t == 0x0003:
+                if flags & 0x02:  # the message holds a reference to a committed type
+                    raise HDF5Error(f"{self.name}: committed (shared) datatypes are not supported")
+                return _parse_type(_Buf(raw))
+        raise HDF5Error(f"{self.name}: no datatype")
 
     def __getitem__(self, key):
         if key != () and key != Ellipsis:
@@ -560,7 +565,9 @@ def _decode(f: H5File, raw: bytes, t: _Type, count: int, as_str: bool = True):
 def _parse_attribute(f: H5File, raw: bytes):
     b = _Buf(raw)
     version = b.u(1)
-    b.skip(1)  # reserved (v1) / flags (v2, v3)
+    flags = b.u(1)  # reserved (v1) / flags (v2, v3)
+    if version >= 2 and flags & 0x03:
+        raise HDF5Error("attributes with shared datatypes or dataspaces are not supported")
     name_size, type_size, space_size = b.u(2), b.u(2), b.u(2)
     if version == 3:
         b.skip(1)  # name character set

@@ -20,8 +20,9 @@ Files (small, committed):
                           big-endian integer array, a compact dataset, a
                           fixed-length string attribute, a variable-length
                           string dataset, a scalar, 40 members in one group
-                          (several symbol-table nodes) and 30 attributes on
-                          one dataset (object header continuation).
+                          (several symbol-table nodes), 30 attributes on
+                          one dataset (object header continuation) and a
+                          dataset of a committed datatype (rejected).
   hdf5_expected.npz       every array above (allow_pickle=False)
   hdf5_expected.json      every string / attribute / member list above
 """
@@ -116,6 +117,8 @@ with h5py.File(os.path.join(HERE, "hdf5_formats.h5"), "w") as f:
     d = f.create_dataset("attrs30", data=np.arange(5.0))
     for i in range(30):
         d.attrs[f"a{i:02d}"] = i * 1.5
+    f["committed_f4"] = np.dtype("<f4")  # a named (committed) datatype
+    f.create_dataset("uses_committed_type", data=np.ones(3, "<f4"), dtype=f["committed_f4"])
 arrays.update({"chunked_gzip": chunked, "chunked_shuffle_gzip": shuffled,
                "big_endian": big_endian.astype(np.uint16), "compact": compact})
 arrays.update({f"many/{k}": v for k, v in many.items()})

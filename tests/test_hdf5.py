@@ -67,6 +67,8 @@ def test_storage_forms():
         assert f["big_endian"].attrs == {"fixed": META["fixed_attr"]}
         assert f["attrs30"].attrs == META["attrs30"]
         assert f["many"].keys() == META["many"]
+        with pytest.raises(HDF5Error, match="committed"):
+            f["uses_committed_type"][()]
         for k in META["many"]:
             np.testing.assert_array_equal(f[f"many/{k}"][()], EXP[f"many__{k}"])
 
