@@ -345,13 +345,18 @@ struct WinKnots {
 // the first guess, 4 above the last) put li = k0 - k_lo in [2, nk - 5].
 // The search then ends on the exact segment w_l <= x < w_r, whose width
 // is positive.
-template <bool kKappa, bool kInterior = false>
-__device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, int k_lo, int nk,
-                                     const float* lam) {
-  const int n = g.n_wave;
+// Locator guess k0 of target x: the compact locator copy (4 bytes per cell)
+// holds the number of knots <= the cell start, minus one.
+__device__ inline int loc_k0(const SspView& g, float x, const Shift& h) {
   const float fm = __builtin_amdgcn_fmed3f(fmaf(x, h.c1, h.c0), 0.f, float(g.n_loc - 1));
-  // compact locator copy (4 bytes per cell): k0 = knots <= cell start - 1
-  const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
+  return ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
+}
+
+// win_knots with the locator guess already loaded (k0 = loc_k0(g, x, h)).
+template <bool kKappa, bool kInterior = false>
+__device__ inline WinKnots win_knots_k0(const SspView& g, float x, const Shift& h, int k_lo,
+                                        int nk, const float* lam, int k0) {
+  const int n = g.n_wave;
   const float y = x * h.inv_a;
   const int li_max = nk - 2, li_min = 0;
   int li = k0 - k_lo;
@@ -390,6 +395,56 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
   K.kappa = 0.f;
   if (kKappa) K.kappa = (left || right || flat) ? 0.f : (l0 + th * (l1 - l0)) * r;
   return K;
+}
+
+template <bool kKappa, bool kInterior = false>
+__device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, int k_lo, int nk,
+                                     const float* lam) {
+  return win_knots_k0<kKappa, kInterior>(g, x, h, k_lo, nk, lam, loc_k0(g, x, h));
+}
+
+// P_j of NB bins of a lane in an interior window (the conditions of
+// win_knots<false, true>), the bins interleaved: the same operations per
+// bin as win_knots_k0 followed by S0 + theta (S1 - S0) on the staged
+// S_k, so the same values, with the LDS reads of all bins in flight
+// together and one wave vote for the rare correction.
+template <int NB>
+__device__ inline void win_values_interior(const float (&x)[NB], const int (&k0)[NB],
+                                           const Shift& h, int k_lo, int nk, const float* lam,
+                                           const float* S, float (&P)[NB]) {
+  const int li_max = nk - 2;
+  int li[NB];
+  float wl[NB], wr[NB];
+  bool any = false;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const float y = x[t] * h.inv_a;
+    int l = k0[t] - k_lo;
+    l += lam[l + 1] <= y ? 1 : 0;
+    li[t] = l;
+    wl[t] = mul_nocontract(lam[l], h.a);
+    wr[t] = mul_nocontract(lam[l + 1], h.a);
+    any = any || (x[t] >= wr[t] && l < li_max) || (x[t] < wl[t] && l > 0);
+  }
+  if (__ballot(any)) {
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      bool step = (x[t] >= wr[t] && li[t] < li_max) || (x[t] < wl[t] && li[t] > 0);
+      while (step) {
+        li[t] += x[t] >= wr[t] ? 1 : -1;
+        wl[t] = mul_nocontract(lam[li[t]], h.a);
+        wr[t] = mul_nocontract(lam[li[t] + 1], h.a);
+        step = (x[t] >= wr[t] && li[t] < li_max) || (x[t] < wl[t] && li[t] > 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const float r = __builtin_amdgcn_rcpf(wr[t] - wl[t]);
+    const float th = (x[t] - wl[t]) * r;
+    const float S0 = S[li[t]], S1 = S[li[t] + 1];
+    P[t] = S0 + th * (S1 - S0);
+  }
 }
 
 // Locator guess of the left knot of x's segment: k0 of x's cell plus the
@@ -656,8 +711,10 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
       const uint32_t j = jh + lane + 64u * t;
       const uint32_t jc = j < j1 ? j : (j1 > j0 ? j1 - 1 : 0);
       const Pair xd = ld_pair(tg.pairs, 2 * jc);
+      // a bin past the end takes the last bin's x (inside the window) and
+      // weight 0, so it adds nothing to Nn and its sum is never stored
       x[t] = xd.lo;
-      dt[t] = xd.hi;
+      dt[t] = j < jh_end ? xd.hi : 0.f;
       tau[t] = kDust ? dust_tau(axav, jc) : 0.f;
       acc[t] = 0.0;
     }
@@ -674,6 +731,10 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
         int k_lo, nk;
         slab_window(g, tg, jh, jh_end, h, &k_lo, &nk);
         if (nk <= kHalfWindow) {
+          // locator guesses first: their loads overlap the staging
+          int k0[kFusedBins];
+#pragma unroll
+          for (int t = 0; t < kFusedBins; ++t) k0[t] = loc_k0(g, x[t], h);
           for (int k = lane; k < nk; k += 64) {
             const uint32_t kk = uint32_t(k_lo + k);
             lam[k] = ld_at(g.wave, kk);
@@ -683,19 +744,17 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
             F[k] = f * s.m;  // S_k = F_k m, rounded once per knot as the reference's spectra
           }
           wave_lds_sync();
-          auto bins = [&](auto interior) {
+          // every bin of the lane, also one past the end (its x lies in the window)
+          if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) {
+            win_values_interior<kFusedBins>(x, k0, h, k_lo, nk, lam, F, P);
+          } else {
 #pragma unroll
             for (int t = 0; t < kFusedBins; ++t) {
-              if (jh + lane + 64u * t < jh_end) {
-                const WinKnots K =
-                    win_knots<false, decltype(interior)::value>(g, x[t], h, k_lo, nk, lam);
-                const float S0 = F[K.li], S1 = F[K.li + 1];
-                P[t] = S0 + K.theta * (S1 - S0);
-              }
+              const WinKnots K = win_knots_k0<false>(g, x[t], h, k_lo, nk, lam, k0[t]);
+              const float S0 = F[K.li], S1 = F[K.li + 1];
+              P[t] = S0 + K.theta * (S1 - S0);
             }
-          };
-          if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) bins(std::true_type{});
-          else bins(std::false_type{});
+          }
           wave_lds_sync();  // this window's reads done before the next star's stores
         } else {
 #pragma unroll
@@ -709,8 +768,7 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
       }
       float nn = 0.f;
 #pragma unroll
-      for (int t = 0; t < kFusedBins; ++t)
-        if (jh + lane + 64u * t < jh_end) nn += P[t] * dt[t];
+      for (int t = 0; t < kFusedBins; ++t) nn += P[t] * dt[t];
       nn = wave_sum(nn);
       if (lane == 0) nn_s[par][wave] = nn;
       __syncthreads();  // every share of this star posted
@@ -727,11 +785,9 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
       }
 #pragma unroll
       for (int t = 0; t < kFusedBins; ++t) {
-        if (jh + lane + 64u * t < jh_end) {
-          float v = P[t] * sf;
-          if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
-          acc[t] += double(v);
-        }
+        float v = P[t] * sf;
+        if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
+        acc[t] += double(v);
       }
     }
 #pragma unroll
