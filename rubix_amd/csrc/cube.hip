@@ -1025,9 +1025,12 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
       }
       auto groups = [&](auto interior_c) {
         constexpr bool kIn = decltype(interior_c)::value;
-      for (uint32_t jb = j0; jb < j1; jb += 64) {
+      // a full group (every lane's bin below j1) runs without the per-lane
+      // validity selects and the masked cotangent loads
+      auto group = [&](uint32_t jb, auto full_c) {
+        constexpr bool kFull = decltype(full_c)::value;
         const uint32_t j = jb + uint32_t(t.lane);
-        const bool valid = j < j1;
+        const bool valid = kFull || j < j1;
         const uint32_t jc = valid ? j : j1 - 1;
         const Pair xd = ld_pair(tg.pairs, 2 * jc);
         const float x = xd.lo;
@@ -1123,7 +1126,10 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
           }
         }
         if (gather && valid) bwd_bin_gather(g, R, s, B, h, x, gj, dj, v);
-      }
+      };
+      uint32_t jb = j0;
+      for (; jb + 64 <= j1; jb += 64) group(jb, std::true_type{});
+      if (jb < j1) group(jb, std::false_type{});
       };
       if (interior) groups(std::true_type{});
       else groups(std::false_type{});
