@@ -831,26 +831,36 @@ __global__ void cube_combine_kernel(const double* __restrict__ partial,
 // Each lane accumulates its bins in float32; the wave's 64 lane sums are
 // added in double in a fixed order (lds_sum), so the Doppler sums, which
 // cancel strongly across bins, lose nothing beyond the float32 per-bin terms.
-constexpr int kRedStride = 17;  // padded LDS row of one lane's (up to 16) sums
+constexpr int kRedRow = 68;              // LDS floats per value row (64 lanes, 16-byte rows)
+constexpr int kRedFloats = 8 * kRedRow;  // one wavefront's reduction buffer
 
-// v[0..K) of all 64 lanes -> lane 4q (q < K) holds the wave total of v[q]
-// (double).  The order is fixed: 16 consecutive lanes in order, then pairs
-// of those.  buf is the wave's own LDS (64 * kRedStride floats).
+// v[0..K) of all 64 lanes -> lane 8q (q < K) holds the wave total of v[q]
+// (double).  Lane l stores v[q] at row q, column l; lane 8q + p adds the
+// row's columns 8p .. 8p + 7 in order (two 16-byte LDS reads), then the
+// eight parts are combined by xor-shuffles over 1, 2 and 4: a fixed order.
+// buf is the wave's own LDS (kRedFloats floats, 16-byte aligned).
 template <int K>
 __device__ inline double lds_sum(const float v[K], float* buf, int lane) {
-  static_assert(K <= 16, "at most 16 values");
+  static_assert(K <= 8, "at most 8 values");
   wave_lds_sync();
 #pragma unroll
-  for (int q = 0; q < K; ++q) buf[lane * kRedStride + q] = v[q];
+  for (int q = 0; q < K; ++q) buf[q * kRedRow + lane] = v[q];
   wave_lds_sync();
-  const int q = lane >> 2, p = lane & 3;
-  double acc = 0.0;
-  if (q < K) {
-#pragma unroll
-    for (int t = 0; t < 16; ++t) acc += double(buf[(p * 16 + t) * kRedStride + q]);
-  }
+  const int q = lane >> 3, p = lane & 7;
+  // lanes past the K rows read row 0 (their totals are not used)
+  const float4* src = reinterpret_cast<const float4*>(buf + (q < K ? q : 0) * kRedRow + 8 * p);
+  const float4 a = src[0], b = src[1];
+  double acc = double(a.x);
+  acc += double(a.y);
+  acc += double(a.z);
+  acc += double(a.w);
+  acc += double(b.x);
+  acc += double(b.y);
+  acc += double(b.z);
+  acc += double(b.w);
   acc += __shfl_xor(acc, 1, 64);
   acc += __shfl_xor(acc, 2, 64);
+  acc += __shfl_xor(acc, 4, 64);
   return acc;
 }
 
@@ -872,7 +882,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
     SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
     const float* __restrict__ gradw, const float* __restrict__ dcube, int slab_bins,
     const float* __restrict__ axav, float* __restrict__ vpart, double* __restrict__ apart) {
-  __shared__ float red[W][64 * kRedStride];
+  __shared__ __align__(16) float red[W][kRedFloats];
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const SlabTask t = slab_task<W>(item);
     const bool live = t.r < n;
@@ -928,8 +938,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
       }
     }
     const double tot = lds_sum<7>(v, red[t.wave], t.lane);
-    const int q = t.lane >> 2;
-    if (live && (t.lane & 3) == 0 && q < 7) {
+    const int q = t.lane >> 3;
+    if (live && (t.lane & 7) == 0 && q < 7) {
       const int64_t base = (t.r * kSlabs + t.slab) * 8;
       if (q < 5) vpart[base + q] = float(tot);
       else apart[base + q - 5] = tot;
@@ -991,7 +1001,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
     SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
     const float* __restrict__ gradw, const float* __restrict__ dcube, int slab_bins,
     const float* __restrict__ axav, float* __restrict__ vpart, double* __restrict__ apart) {
-  __shared__ float red[W][64 * kRedStride];
+  __shared__ __align__(16) float red[W][kRedFloats];
   const int n_w = g.n_wave;
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const SlabTask t = slab_task<W>(item);
@@ -1135,8 +1145,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
       else groups(std::false_type{});
     }
     const double tot = lds_sum<7>(v, red[t.wave], t.lane);
-    const int q = t.lane >> 2;
-    if (live && (t.lane & 3) == 0 && q < 7) {
+    const int q = t.lane >> 3;
+    if (live && (t.lane & 7) == 0 && q < 7) {
       const int64_t base = (t.r * kSlabs + t.slab) * 8;
       if (q < 5) vpart[base + q] = float(tot);
       else apart[base + q - 5] = tot;
