@@ -685,6 +685,25 @@ constexpr int kLaneBins = 8;
 // value is computed once instead of twice.  The first wavefront stores sf
 // and Nn for the adjoint.
 constexpr int kFusedWaves = 2 * kSlabs;
+static_assert(kFusedWaves == 16, "the Nn shares are added as one 16-lane row");
+
+// v + the row-of-16 DPP move of v (every lane of the wave active).
+template <int kCtrl>
+__device__ inline float plus_dpp(float v) {
+  return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+
+// Sum of the 16 values held by lanes 0..15 (lanes 16..63 hold copies of
+// them), the same fixed tree in every wavefront: pairs (xor 1), quads
+// (xor 2), then rotations of the row by 4 and 8; lane 0's total, broadcast.
+// All 64 lanes must be active.
+__device__ inline float sum16_row0(float v) {
+  v = plus_dpp<0xB1>(v);   // quad_perm(1, 0, 3, 2)
+  v = plus_dpp<0x4E>(v);   // quad_perm(2, 3, 0, 1)
+  v = plus_dpp<0x124>(v);  // row_ror:4
+  v = plus_dpp<0x128>(v);  // row_ror:8
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 constexpr int kFusedBins = kLaneBins / 2;  // bins per lane
 constexpr int kHalfWindow = 320;           // knots per staged half-slab window
 
@@ -779,9 +798,7 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
       nn = wave_sum(nn);
       if (lane == 0) nn_s[par][wave] = nn;
       __syncthreads();  // every share of this star posted
-      float Nn = 0.f;
-#pragma unroll
-      for (int b = 0; b < kFusedWaves; ++b) Nn += nn_s[par][b];
+      const float Nn = sum16_row0(nn_s[par][lane & 15]);
       par ^= 1;
       float sf = s.T / Nn;
       if (isnan(sf)) sf = 0.f;
