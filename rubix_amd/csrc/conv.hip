@@ -9,9 +9,12 @@
 // convolution along the middle axis of an (n_outer, n_axis, n_inner) view.
 // Work is split so that index arithmetic happens once per workgroup item,
 // never per element (64-bit division per element would dominate):
-// * conv_lines_kernel (n_inner > 1, the spatial PSF passes): item = one
-//   (outer, axis) line and a chunk of up to kLineChunk contiguous inner
-//   elements; threads run along the contiguous inner axis.
+// * conv_tile_kernel (n_inner > 1 and an axis of at most kTileAxis, the
+//   spatial PSF passes): item = a block of 64 contiguous inner elements with
+//   the whole axis staged in LDS, so each input is read once.
+// * conv_lines_kernel (n_inner > 1, longer axes): item = one (outer, axis)
+//   line and a chunk of up to kLineChunk contiguous inner elements; threads
+//   run along the contiguous inner axis.
 // * conv_rows_kernel (n_inner == 1, the LSF along lambda): item = a segment
 //   of kSeg outputs of one row; the segment's inputs plus the kernel's halo
 //   are staged in LDS once and every output reads its taps from LDS.
@@ -69,6 +72,7 @@ constexpr int kDotBlocksMax = 4096;
 constexpr int kLineChunk = 1024;  // inner elements per line item
 constexpr int kSeg = 2048;        // outputs per row item of the LSF
 constexpr int kHalo = 256;        // taps - 1 supported by the row kernel
+constexpr int kTileAxis = 240;    // longest axis the tiled line kernels stage (60 KB of LDS)
 
 // One 1D convolution item of a line: out[o, x, i0 + k] for k < len.
 template <class Acc, class Tap>
@@ -102,6 +106,36 @@ __global__ void __launch_bounds__(256) conv_lines_kernel(
   }
 }
 
+// Items: (o, block of 64 contiguous inner elements).  The workgroup stages
+// the block's whole axis (n_axis x 64 floats, dynamic LDS) once with
+// coalesced loads; wavefront w then forms the outputs x = w, w + 4, ...
+// from LDS, the taps in the order t = 0 .. m-1 of line_tap_sum (same
+// values).  Every input element is read from memory once instead of once
+// per tap.  n_axis <= kTileAxis.
+__global__ void __launch_bounds__(256) conv_tile_kernel(
+    const float* __restrict__ in, float* __restrict__ out, int n_axis, int64_t n_inner,
+    const float* __restrict__ taps, int n_taps, int start, int64_t n_blocks, int64_t n_items) {
+  extern __shared__ float tile[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t o = item / n_blocks;
+    const int64_t i = (item - o * n_blocks) * 64 + lane;
+    const bool ok = i < n_inner;
+    const int64_t base = o * n_axis * n_inner + (ok ? i : 0);
+    __syncthreads();  // the previous item's reads are done
+    for (int x = w; x < n_axis; x += 4) tile[x * 64 + lane] = ok ? in[base + int64_t(x) * n_inner] : 0.f;
+    __syncthreads();
+    for (int x = w; x < n_axis; x += 4) {
+      float acc = 0.f;
+      for (int t = 0; t < n_taps; ++t) {
+        const int xs = x + start - t;
+        if (xs >= 0 && xs < n_axis) acc += taps[t] * tile[xs * 64 + lane];
+      }
+      if (ok) out[base + int64_t(x) * n_inner] = acc;
+    }
+  }
+}
+
 // Items: (row o, segment) of a row-contiguous convolution (n_inner == 1).
 // lds[k] = in[o, x0 + start - (m - 1) + k] (0 outside the row), so
 // out[o, x0 + v] = sum_t taps[t] lds[v + m - 1 - t].
@@ -121,12 +155,11 @@ __global__ void __launch_bounds__(256) conv_rows_kernel(
       lds[k] = (xs >= 0 && xs < n_axis) ? row[xs] : 0.f;
     }
     __syncthreads();
+    // lds is zero outside the row, so every tap reads it without a bounds
+    // test: a tap past the row's end adds taps[t] * 0
     for (int v = threadIdx.x; v < kSeg && x0 + v < n_axis; v += blockDim.x) {
       float acc = 0.f;
-      for (int t = 0; t < n_taps; ++t) {
-        const int xs = x0 + v + start - t;
-        if (xs >= 0 && xs < n_axis) acc += taps[t] * lds[v + n_taps - 1 - t];
-      }
+      for (int t = 0; t < n_taps; ++t) acc += taps[t] * lds[v + n_taps - 1 - t];
       out[o * n_axis + x0 + v] = acc;
     }
   }
@@ -187,11 +220,39 @@ __global__ void __launch_bounds__(kDotThreads) conv_dot_rows_kernel(
     __syncthreads();
     for (int v = threadIdx.x; v < kSeg && x0 + v < n_axis; v += blockDim.x) {
       double c = 0.0;
-      for (int t = 0; t < n_taps; ++t) {
-        const int xs = x0 + v + start - t;
-        if (xs >= 0 && xs < n_axis) c += taps[t] * double(lds[v + n_taps - 1 - t]);
-      }
+      for (int t = 0; t < n_taps; ++t) c += taps[t] * double(lds[v + n_taps - 1 - t]);
       acc += double(a[o * n_axis + x0 + v]) * c;
+    }
+  }
+  block_partial(acc, sm, partial);
+}
+
+// sum(a * conv(b)) per block over tiled items (as conv_tile_kernel), double
+// taps.  Each thread adds its outputs' a * conv(b) in item order.
+__global__ void __launch_bounds__(kDotThreads) conv_dot_tile_kernel(
+    const float* __restrict__ a, const float* __restrict__ b, int n_axis, int64_t n_inner,
+    const double* __restrict__ taps, int n_taps, int start, int64_t n_blocks, int64_t n_items,
+    double* __restrict__ partial) {
+  extern __shared__ float tile[];
+  __shared__ double sm[kDotThreads];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t o = item / n_blocks;
+    const int64_t i = (item - o * n_blocks) * 64 + lane;
+    const bool ok = i < n_inner;
+    const int64_t base = o * n_axis * n_inner + (ok ? i : 0);
+    __syncthreads();
+    for (int x = w; x < n_axis; x += 4) tile[x * 64 + lane] = ok ? b[base + int64_t(x) * n_inner] : 0.f;
+    __syncthreads();
+    if (!ok) continue;
+    for (int x = w; x < n_axis; x += 4) {
+      double c = 0.0;
+      for (int t = 0; t < n_taps; ++t) {
+        const int xs = x + start - t;
+        if (xs >= 0 && xs < n_axis) c += taps[t] * double(tile[xs * 64 + lane]);
+      }
+      acc += double(a[base + int64_t(x) * n_inner]) * c;
     }
   }
   block_partial(acc, sm, partial);
@@ -359,6 +420,12 @@ int conv_axis(const float* in, float* out, int64_t n_outer, int n_axis, int64_t 
     const int64_t items = n_outer * n_seg;
     hipLaunchKernelGGL(conv_rows_kernel, dim3(item_blocks(items, 1 << 20)), dim3(256), 0, s, in,
                        out, n_axis, taps, n_taps, start, n_seg, items);
+  } else if (n_inner > 1 && n_axis <= kTileAxis) {
+    const int64_t n_blocks = (n_inner + 63) / 64;
+    const int64_t items = n_outer * n_blocks;
+    hipLaunchKernelGGL(conv_tile_kernel, dim3(item_blocks(items, 1 << 20)), dim3(256),
+                       size_t(n_axis) * 64 * sizeof(float), s, in, out, n_axis, n_inner, taps,
+                       n_taps, start, n_blocks, items);
   } else if (n_inner > 1) {
     const int64_t n_chunks = (n_inner + kLineChunk - 1) / kLineChunk;
     const int64_t items = n_outer * n_axis * n_chunks;
@@ -431,6 +498,11 @@ int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer, int32_t n
     const int64_t n_seg = (n_axis + kSeg - 1) / kSeg;
     hipLaunchKernelGGL(conv_dot_rows_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_axis,
                        taps, n_taps, start, n_seg, n_outer * n_seg, partial);
+  } else if (n_inner > 1 && n_axis <= kTileAxis) {
+    const int64_t n_blocks = (n_inner + 63) / 64;
+    hipLaunchKernelGGL(conv_dot_tile_kernel, dim3(nblk), dim3(kDotThreads),
+                       size_t(n_axis) * 64 * sizeof(float), s, a, b, n_axis, n_inner, taps, n_taps,
+                       start, n_blocks, n_outer * n_blocks, partial);
   } else if (n_inner > 1) {
     const int64_t n_chunks = (n_inner + kLineChunk - 1) / kLineChunk;
     hipLaunchKernelGGL(conv_dot_lines_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,

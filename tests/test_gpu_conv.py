@@ -5,8 +5,9 @@ separable PSF of rubix/telescope/psf/psf.py:33-58 with the kernel of
 rubix/telescope/psf/kernels.py:5-31, and the 25-tap LSF of
 rubix/telescope/lsf/lsf.py:12-105) against oracle/ref_torch.py in float64
 with torch autograd, at shapes the headline cubes do not reach: a kernel
-wider than the plane, even and one-tap kernels, one-plane cubes, spectra
-shorter than the LSF, one-bin spectra, and more than one LSF row segment.
+wider than the plane, even and one-tap kernels, one-plane cubes, an axis
+too long for the LDS-tiled kernels, spectra shorter than the LSF, one-bin
+spectra, and more than one LSF row segment.
 The general 2D kernel path (conv2d_planes) is checked on the reference's own
 cases (tests/test_telescope_psf.py:22-55: a bright pixel under a 3x3 ones
 kernel, which sums exactly), and the LSF on the reference's delta cases
@@ -48,6 +49,7 @@ PSF_CASES = [  # (ny, nx, nl, size, sigma)
     (10, 10, 9, 2, 0.6),
     (50, 50, 17, 15, 4.0),
     (21, 13, 1100, 11, 1.2),
+    (250, 3, 2, 5, 1.0),  # an axis longer than the LDS-tiled kernels stage
 ]
 
 
