@@ -106,6 +106,25 @@ __global__ void __launch_bounds__(256) conv_lines_kernel(
   }
 }
 
+// Rows u0, u0 + 4, u0 + 8, u0 + 12 of the convolution of a staged tile (rows
+// 0 .. n-1 of 64 lanes): for each output the taps t = 0 .. m-1 in order,
+// rows outside [0, n) skipped (zero padding).  Four independent sums per
+// lane hide the LDS latency; each output's own order is unchanged.
+template <class Acc, class Tap>
+__device__ inline void tile_conv4(const float* tile, int lane, int n, const Tap* __restrict__ taps,
+                                  int m, int start, int u0, Acc acc[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = Acc(0);
+  for (int t = 0; t < m; ++t) {
+    const Acc tv = Acc(taps[t]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int us = u0 + 4 * r + start - t;
+      if (us >= 0 && us < n) acc[r] += tv * Acc(tile[us * 64 + lane]);
+    }
+  }
+}
+
 // Items: (o, block of 64 contiguous inner elements).  The workgroup stages
 // the block's whole axis (n_axis x 64 floats, dynamic LDS) once with
 // coalesced loads; wavefront w then forms the outputs x = w, w + 4, ...
@@ -125,13 +144,14 @@ __global__ void __launch_bounds__(256) conv_tile_kernel(
     __syncthreads();  // the previous item's reads are done
     for (int x = w; x < n_axis; x += 4) tile[x * 64 + lane] = ok ? in[base + int64_t(x) * n_inner] : 0.f;
     __syncthreads();
-    for (int x = w; x < n_axis; x += 4) {
-      float acc = 0.f;
-      for (int t = 0; t < n_taps; ++t) {
-        const int xs = x + start - t;
-        if (xs >= 0 && xs < n_axis) acc += taps[t] * tile[xs * 64 + lane];
+    for (int x0 = w; x0 < n_axis; x0 += 16) {
+      float acc[4];
+      tile_conv4<float>(tile, lane, n_axis, taps, n_taps, start, x0, acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = x0 + 4 * r;
+        if (ok && x < n_axis) out[base + int64_t(x) * n_inner] = acc[r];
       }
-      if (ok) out[base + int64_t(x) * n_inner] = acc;
     }
   }
 }
@@ -245,14 +265,16 @@ __global__ void __launch_bounds__(kDotThreads) conv_dot_tile_kernel(
     __syncthreads();
     for (int x = w; x < n_axis; x += 4) tile[x * 64 + lane] = ok ? b[base + int64_t(x) * n_inner] : 0.f;
     __syncthreads();
-    if (!ok) continue;
-    for (int x = w; x < n_axis; x += 4) {
-      double c = 0.0;
-      for (int t = 0; t < n_taps; ++t) {
-        const int xs = x + start - t;
-        if (xs >= 0 && xs < n_axis) c += taps[t] * double(tile[xs * 64 + lane]);
+    if (ok) {
+      for (int x0 = w; x0 < n_axis; x0 += 16) {
+        double c[4];
+        tile_conv4<double>(tile, lane, n_axis, taps, n_taps, start, x0, c);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int x = x0 + 4 * r;
+          if (x < n_axis) acc += double(a[base + int64_t(x) * n_inner]) * c[r];
+        }
       }
-      acc += double(a[base + int64_t(x) * n_inner]) * c;
     }
   }
   block_partial(acc, sm, partial);
@@ -373,6 +395,228 @@ __global__ void __launch_bounds__(256) conv_batch_kernel(
       out[int64_t(p) * nt + l] = acc;
     }
   }
+}
+
+// LDS-tiled batch passes.  A packed table (galaxy g's rows begin where
+// galaxy g - 1's end, from row 0 to n_rows, every nb >= 1: what a batch
+// layout builds) is convolved line by line: item = (line L, block of 64
+// lambda), L counting the lines of galaxy 0, then galaxy 1, ...; the
+// workgroup stages the line's nb x 64 inputs in LDS once (lines of at most
+// kBatchTileAxis spaxels; longer ones read the inputs directly) and adds
+// the taps in the order of conv_batch_kernel, so the values are the same.
+// Each workgroup first builds the line prefix of the table in LDS and
+// checks that it is packed; any other table runs the row items of
+// conv_batch_kernel (rows of no galaxy get zeros).
+constexpr int kBatchTileAxis = 128;
+constexpr int kBatchGalMax = 2048;  // galaxies the line prefix holds (host falls back above)
+
+// lines[g] = number of lines of galaxies 0 .. g-1 (lines[n_gal] = total) and
+// whether the table is packed; all threads of the workgroup return the same.
+__device__ inline bool batch_line_prefix(const rbx_galaxy* __restrict__ gal, int n_gal,
+                                         int n_rows, int* lines, int* part) {
+  const int tid = threadIdx.x;
+  const int per = (n_gal + 255) / 256;
+  const int g0 = tid * per;
+  int sum = 0, bad = 0;
+  for (int k = 0; k < per; ++k) {
+    const int g = g0 + k;
+    if (g >= n_gal) break;
+    const int64_t nb = gal[g].nb, begin = gal[g].spaxel_begin;
+    const int64_t end = g + 1 < n_gal ? int64_t(gal[g + 1].spaxel_begin) : int64_t(n_rows);
+    if (nb < 1 || begin + nb * nb != end || (g == 0 && begin != 0)) bad = 1;
+    sum += nb > 0 ? int(nb) : 0;
+  }
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {  // inclusive scan of the 256 partial sums
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int k = 0; k < per; ++k) {
+    const int g = g0 + k;
+    if (g >= n_gal) break;
+    lines[g] = run;
+    run += gal[g].nb > 0 ? gal[g].nb : 0;
+  }
+  if (tid == 255) lines[n_gal] = part[255];
+  const bool packed = __syncthreads_or(bad) == 0;
+  return packed;
+}
+
+// Galaxy g of line L of a packed table and the line's first element / stride.
+struct BatchLine {
+  int nb;
+  int64_t base, stride;
+};
+
+__device__ inline BatchLine batch_line(const rbx_galaxy* __restrict__ gal, const int* lines,
+                                       int n_gal, int nt, int axis, int L) {
+  int lo = 0, hi = n_gal;  // last galaxy with lines[g] <= L
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (lines[mid] <= L) lo = mid; else hi = mid;
+  }
+  const int j = L - lines[lo];
+  BatchLine b;
+  b.nb = gal[lo].nb;
+  const int64_t begin = gal[lo].spaxel_begin;
+  if (axis == 1) {  // along x: line y = j
+    b.base = (begin + int64_t(j) * b.nb) * nt;
+    b.stride = nt;
+  } else {          // along y: line x = j
+    b.base = (begin + j) * int64_t(nt);
+    b.stride = int64_t(b.nb) * nt;
+  }
+  return b;
+}
+
+// conv_batch_kernel's row item (row p, lambda chunk c) for tables that are
+// not packed.
+__device__ inline void batch_row_item(const float* __restrict__ in, float* __restrict__ out,
+                                      const rbx_galaxy* __restrict__ gal, int n_gal, int n_rows,
+                                      int nt, int axis, const float* __restrict__ taps,
+                                      int n_taps, int start, int n_chunks, int64_t item) {
+  const int p = int(item / n_chunks);
+  const int l0 = int(item - int64_t(p) * n_chunks) * kLineChunk;
+  const int l1 = l0 + kLineChunk < nt ? l0 + kLineChunk : nt;
+  const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, int64_t(p) * nt);
+  for (int l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+    float acc = 0.f;
+    if (b.ok) {
+      for (int t = 0; t < n_taps; ++t) {
+        const int us = b.u + start - t;
+        if (us >= 0 && us < b.nb) acc += taps[t] * in[b.base + l + us * b.stride];
+      }
+    }
+    out[int64_t(p) * nt + l] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(256) conv_batch_tile_kernel(
+    const float* __restrict__ in, float* __restrict__ out, const rbx_galaxy* __restrict__ gal,
+    int n_gal, int n_rows, int nt, int axis, const float* __restrict__ taps, int n_taps,
+    int start) {
+  extern __shared__ float tile[];
+  __shared__ int lines[kBatchGalMax + 1];
+  __shared__ int part[256];
+  const bool packed = batch_line_prefix(gal, n_gal, n_rows, lines, part);
+  if (!packed) {
+    const int n_chunks = (nt + kLineChunk - 1) / kLineChunk;
+    const int64_t n_items = int64_t(n_rows) * n_chunks;
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x)
+      batch_row_item(in, out, gal, n_gal, n_rows, nt, axis, taps, n_taps, start, n_chunks, item);
+    return;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n_lblk = (nt + 63) / 64;
+  const int64_t n_items = int64_t(lines[n_gal]) * n_lblk;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int L = int(item / n_lblk);
+    const int l = int(item - int64_t(L) * n_lblk) * 64 + lane;
+    const bool lok = l < nt;
+    const BatchLine b = batch_line(gal, lines, n_gal, nt, axis, L);
+    const int nb = b.nb;
+    const int64_t e0 = b.base + (lok ? l : 0);
+    if (nb <= kBatchTileAxis) {
+      __syncthreads();  // the previous line's reads are done
+      for (int u = w; u < nb; u += 4) tile[u * 64 + lane] = lok ? in[e0 + u * b.stride] : 0.f;
+      __syncthreads();
+      if (lok) {
+        for (int u0 = w; u0 < nb; u0 += 16) {
+          float acc[4];
+          tile_conv4<float>(tile, lane, nb, taps, n_taps, start, u0, acc);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int u = u0 + 4 * r;
+            if (u < nb) out[e0 + u * b.stride] = acc[r];
+          }
+        }
+      }
+    } else if (lok) {
+      for (int u = w; u < nb; u += 4) {
+        float acc = 0.f;
+        for (int t = 0; t < n_taps; ++t) {
+          const int us = u + start - t;
+          if (us >= 0 && us < nb) acc += taps[t] * in[e0 + us * b.stride];
+        }
+        out[e0 + u * b.stride] = acc;
+      }
+    }
+  }
+}
+
+// sum(a * conv(b)) per block: line items of a packed table as
+// conv_batch_tile_kernel, else the row items of conv_batch_dot_partial_kernel.
+__global__ void __launch_bounds__(kDotThreads) conv_batch_dot_tile_kernel(
+    const float* __restrict__ a, const float* __restrict__ bsrc,
+    const rbx_galaxy* __restrict__ gal, int n_gal, int n_rows, int nt, int axis,
+    const double* __restrict__ taps, int n_taps, int start, double* __restrict__ partial) {
+  extern __shared__ float tile[];
+  __shared__ int lines[kBatchGalMax + 1];
+  __shared__ int part[256];
+  __shared__ double sm[kDotThreads];
+  const bool packed = batch_line_prefix(gal, n_gal, n_rows, lines, part);
+  double acc = 0.0;
+  if (!packed) {
+    const int n_chunks = (nt + kLineChunk - 1) / kLineChunk;
+    const int64_t n_items = int64_t(n_rows) * n_chunks;
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+      const int p = int(item / n_chunks);
+      const int l0 = int(item - int64_t(p) * n_chunks) * kLineChunk;
+      const int l1 = l0 + kLineChunk < nt ? l0 + kLineChunk : nt;
+      const BatchPos b = batch_pos(gal, n_gal, n_rows, nt, axis, int64_t(p) * nt);
+      if (!b.ok) continue;
+      for (int l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+        double c = 0.0;
+        for (int t = 0; t < n_taps; ++t) {
+          const int us = b.u + start - t;
+          if (us >= 0 && us < b.nb) c += taps[t] * double(bsrc[b.base + l + us * b.stride]);
+        }
+        acc += double(a[int64_t(p) * nt + l]) * c;
+      }
+    }
+  } else {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n_lblk = (nt + 63) / 64;
+    const int64_t n_items = int64_t(lines[n_gal]) * n_lblk;
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+      const int L = int(item / n_lblk);
+      const int l = int(item - int64_t(L) * n_lblk) * 64 + lane;
+      const bool lok = l < nt;
+      const BatchLine b = batch_line(gal, lines, n_gal, nt, axis, L);
+      const int nb = b.nb;
+      const int64_t e0 = b.base + (lok ? l : 0);
+      if (nb <= kBatchTileAxis) {
+        __syncthreads();
+        for (int u = w; u < nb; u += 4) tile[u * 64 + lane] = lok ? bsrc[e0 + u * b.stride] : 0.f;
+        __syncthreads();
+        if (lok) {
+          for (int u0 = w; u0 < nb; u0 += 16) {
+            double c[4];
+            tile_conv4<double>(tile, lane, nb, taps, n_taps, start, u0, c);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int u = u0 + 4 * r;
+              if (u < nb) acc += double(a[e0 + u * b.stride]) * c[r];
+            }
+          }
+        }
+      } else if (lok) {
+        for (int u = w; u < nb; u += 4) {
+          double c = 0.0;
+          for (int t = 0; t < n_taps; ++t) {
+            const int us = u + start - t;
+            if (us >= 0 && us < nb) c += taps[t] * double(bsrc[e0 + us * b.stride]);
+          }
+          acc += double(a[e0 + u * b.stride]) * c;
+        }
+      }
+    }
+  }
+  block_partial(acc, sm, partial);
 }
 
 __global__ void __launch_bounds__(kDotThreads) conv_batch_dot_partial_kernel(
@@ -526,6 +770,13 @@ int rbx_conv_batch(const float* in, float* out, const rbx_galaxy* galaxies, int3
   if (in == out) return fail(RBX_EINVAL, "rbx_conv_batch: in-place convolution not supported");
   const int64_t total = int64_t(n_rows) * n_target;
   if (total == 0) return RBX_OK;
+  if (n_gal <= kBatchGalMax) {
+    // persistent grid: every workgroup builds the table's line prefix once
+    hipLaunchKernelGGL(conv_batch_tile_kernel, dim3(2048), dim3(256),
+                       size_t(kBatchTileAxis) * 64 * sizeof(float), as_stream(stream), in, out,
+                       galaxies, n_gal, n_rows, n_target, axis, taps, n_taps, start);
+    return check_launch("rbx_conv_batch");
+  }
   const int n_chunks = (n_target + kLineChunk - 1) / kLineChunk;
   const int64_t items = int64_t(n_rows) * n_chunks;
   hipLaunchKernelGGL(conv_batch_kernel, dim3(item_blocks(items, 1 << 20)), dim3(256), 0,
@@ -548,10 +799,16 @@ int rbx_conv_batch_dot(const float* a, const float* b, const rbx_galaxy* galaxie
     return fail(RBX_EWORKSPACE, "rbx_conv_batch_dot: workspace too small");
   hipStream_t s = as_stream(stream);
   double* partial = static_cast<double*>(workspace);
-  const int n_chunks = (n_target + kLineChunk - 1) / kLineChunk;
-  hipLaunchKernelGGL(conv_batch_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b,
-                     galaxies, n_gal, n_rows, n_target, axis, taps, n_taps, start, n_chunks,
-                     int64_t(n_rows) * n_chunks, partial);
+  if (n_gal <= kBatchGalMax) {
+    hipLaunchKernelGGL(conv_batch_dot_tile_kernel, dim3(nblk), dim3(kDotThreads),
+                       size_t(kBatchTileAxis) * 64 * sizeof(float), s, a, b, galaxies, n_gal,
+                       n_rows, n_target, axis, taps, n_taps, start, partial);
+  } else {
+    const int n_chunks = (n_target + kLineChunk - 1) / kLineChunk;
+    hipLaunchKernelGGL(conv_batch_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b,
+                       galaxies, n_gal, n_rows, n_target, axis, taps, n_taps, start, n_chunks,
+                       int64_t(n_rows) * n_chunks, partial);
+  }
   hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, nblk, out);
   return check_launch("rbx_conv_batch_dot");
 }

@@ -149,3 +149,65 @@ def test_lsf_reference_delta_cases():
     for i, p in enumerate((20, 50, 75)):
         e = np.exp(-0.5 * (x - p) ** 2 / 2.0 ** 2)
         np.testing.assert_allclose(got[i], e / e.sum(), atol=1e-5)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_batched_psf_pass_equals_per_galaxy(axis):
+    """rbx_conv_batch over a ragged batch (nb 1, 7, 130, 3: a one-spaxel
+    grid, and a line longer than the batch kernel stages in LDS) against the
+    single-galaxy pass on each galaxy's own (nb, nb, n_lambda) cube: the same
+    taps in the same order, so bit for bit; the sigma dot (f64, another
+    summation order) to 1e-12."""
+    nbs, nt, m = [1, 7, 130, 3], 70, 15
+    layout = ops.BatchLayout([1] * len(nbs), nbs, DEV)
+    rng = np.random.default_rng(40 + axis)
+    x = torch.tensor(rng.normal(size=(layout.n_rows, nt)), dtype=torch.float32, device=DEV)
+    y = torch.tensor(rng.normal(size=(layout.n_rows, nt)), dtype=torch.float32, device=DEV)
+    taps = torch.tensor(rng.uniform(0.0, 1.0, m), dtype=torch.float32, device=DEV)
+    dtaps = torch.tensor(rng.normal(size=m), dtype=torch.float64, device=DEV)
+    start = (m - 1) // 2
+    got = ops.conv_batch(x, layout, axis, taps, start).cpu().numpy()
+    dot = float(ops.conv_batch_dot(y, x, layout, axis, dtaps, start))
+    want_dot = 0.0
+    for g, nb in enumerate(nbs):
+        r0 = int(layout.row_begin[g])
+        xg = x[r0:r0 + nb * nb].reshape(nb, nb, nt).contiguous()
+        yg = y[r0:r0 + nb * nb].reshape(nb, nb, nt).contiguous()
+        if axis == 1:
+            ref = ops.conv_axis(xg, taps, nb, nb, nt, start)
+            want_dot += float(ops.conv_axis_dot(yg, xg, dtaps, nb, nb, nt, start))
+        else:
+            ref = ops.conv_axis(xg, taps, 1, nb, nb * nt, start)
+            want_dot += float(ops.conv_axis_dot(yg, xg, dtaps, 1, nb, nb * nt, start))
+        np.testing.assert_array_equal(got[r0:r0 + nb * nb], ref.reshape(nb * nb, nt).cpu().numpy(),
+                                      err_msg=f"galaxy {g} (nb {nb}), axis {axis}")
+    assert abs(dot - want_dot) <= 1e-12 * max(1.0, abs(want_dot))
+
+
+def test_batched_psf_table_with_a_gap_row():
+    """A galaxy table that is not packed (one row between the two galaxies,
+    which belongs to neither) takes the row items: the gap row is zero and
+    each galaxy's rows equal its single-galaxy pass bit for bit."""
+    from types import SimpleNamespace
+    from rubix_amd._lib import RbxGalaxy
+    nbs, begins, nt, m = [5, 4], [0, 26], 40, 7
+    n_rows = 26 + 16
+    arr = (RbxGalaxy * 2)()
+    for g in range(2):
+        arr[g].star_begin = 0
+        arr[g].spaxel_begin = begins[g]
+        arr[g].nb = nbs[g]
+    table = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).to(DEV)
+    layout = SimpleNamespace(table=table, n_gal=2, n_rows=n_rows)
+    rng = np.random.default_rng(77)
+    x = torch.tensor(rng.normal(size=(n_rows, nt)), dtype=torch.float32, device=DEV)
+    taps = torch.tensor(rng.uniform(0.0, 1.0, m), dtype=torch.float32, device=DEV)
+    for axis in (0, 1):
+        got = ops.conv_batch(x, layout, axis, taps, 3).cpu().numpy()
+        assert np.all(got[25] == 0.0)
+        for g, nb in enumerate(nbs):
+            xg = x[begins[g]:begins[g] + nb * nb].reshape(nb, nb, nt).contiguous()
+            ref = (ops.conv_axis(xg, taps, nb, nb, nt, 3) if axis == 1
+                   else ops.conv_axis(xg, taps, 1, nb, nb * nt, 3))
+            np.testing.assert_array_equal(got[begins[g]:begins[g] + nb * nb],
+                                          ref.reshape(nb * nb, nt).cpu().numpy())
