@@ -407,8 +407,8 @@ __global__ void __launch_bounds__(256) conv_batch_kernel(
 // Each workgroup first builds the line prefix of the table in LDS and
 // checks that it is packed; any other table runs the row items of
 // conv_batch_kernel (rows of no galaxy get zeros).
-constexpr int kBatchTileAxis = 128;
-constexpr int kBatchGalMax = 2048;  // galaxies the line prefix holds (host falls back above)
+constexpr int kBatchTileAxis = 100;
+constexpr int kBatchGalMax = 512;  // galaxies the line prefix holds (host falls back above)
 
 // lines[g] = number of lines of galaxies 0 .. g-1 (lines[n_gal] = total) and
 // whether the table is packed; all threads of the workgroup return the same.

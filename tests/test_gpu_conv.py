@@ -153,12 +153,12 @@ def test_lsf_reference_delta_cases():
 
 @pytest.mark.parametrize("axis", [0, 1])
 def test_batched_psf_pass_equals_per_galaxy(axis):
-    """rbx_conv_batch over a ragged batch (nb 1, 7, 130, 3: a one-spaxel
+    """rbx_conv_batch over a ragged batch (nb 1, 7, 101, 3: a one-spaxel
     grid, and a line longer than the batch kernel stages in LDS) against the
     single-galaxy pass on each galaxy's own (nb, nb, n_lambda) cube: the same
     taps in the same order, so bit for bit; the sigma dot (f64, another
     summation order) to 1e-12."""
-    nbs, nt, m = [1, 7, 130, 3], 70, 15
+    nbs, nt, m = [1, 7, 101, 3], 70, 15
     layout = ops.BatchLayout([1] * len(nbs), nbs, DEV)
     rng = np.random.default_rng(40 + axis)
     x = torch.tensor(rng.normal(size=(layout.n_rows, nt)), dtype=torch.float32, device=DEV)
