@@ -232,7 +232,10 @@ int rbx_batch_bwd(const rbx_ssp* ssp, const rbx_target* target,
  * (nb, nb) spaxel grid, cube[y, x, l] layout as rubix/core/psf.py:56-57
  * convolves it): axis 1 convolves along x, axis 0 along y,
  * out[.., u, ..] = sum_t taps[t] in[.., u + start - t, ..] with zero padding
- * inside each galaxy.  rbx_conv_batch_dot returns
+ * inside each galaxy; a row that belongs to no galaxy (a table whose
+ * galaxies are not packed row after row from row 0) is written as zeros.
+ * Packed tables of up to 512 galaxies take an LDS-tiled path with the same
+ * values.  rbx_conv_batch_dot returns
  * out[0] = sum a * conv(b) (double taps, double accumulation, fixed order);
  * its workspace is rbx_dot_workspace_size(n_rows * n_target). */
 int rbx_conv_batch(const float* in, float* out, const rbx_galaxy* galaxies,
