@@ -177,10 +177,20 @@ __global__ void __launch_bounds__(256) conv_rows_kernel(
     __syncthreads();
     // lds is zero outside the row, so every tap reads it without a bounds
     // test: a tap past the row's end adds taps[t] * 0
-    for (int v = threadIdx.x; v < kSeg && x0 + v < n_axis; v += blockDim.x) {
-      float acc = 0.f;
-      for (int t = 0; t < n_taps; ++t) acc += taps[t] * lds[v + n_taps - 1 - t];
-      out[o * n_axis + x0 + v] = acc;
+    // outputs v, v + 256, v + 512, v + 768 of the thread as four independent
+    // sums (each in the order t = 0 .. m-1), hiding the LDS latency
+    for (int v0 = threadIdx.x; v0 < kSeg; v0 += 4 * 256) {
+      float acc[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < n_taps; ++t) {
+        const float tv = taps[t];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] += tv * lds[v0 + 256 * k + n_taps - 1 - t];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int v = v0 + 256 * k;
+        if (v < kSeg && x0 + v < n_axis) out[o * n_axis + x0 + v] = acc[k];
+      }
     }
   }
 }
@@ -238,10 +248,18 @@ __global__ void __launch_bounds__(kDotThreads) conv_dot_rows_kernel(
       lds[k] = (xs >= 0 && xs < n_axis) ? row[xs] : 0.f;
     }
     __syncthreads();
-    for (int v = threadIdx.x; v < kSeg && x0 + v < n_axis; v += blockDim.x) {
-      double c = 0.0;
-      for (int t = 0; t < n_taps; ++t) c += taps[t] * double(lds[v + n_taps - 1 - t]);
-      acc += double(a[o * n_axis + x0 + v]) * c;
+    for (int v0 = threadIdx.x; v0 < kSeg; v0 += 4 * 256) {
+      double c[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int t = 0; t < n_taps; ++t) {
+        const double tv = taps[t];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] += tv * double(lds[v0 + 256 * k + n_taps - 1 - t]);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int v = v0 + 256 * k;
+        if (v < kSeg && x0 + v < n_axis) acc += double(a[o * n_axis + x0 + v]) * c[k];
+      }
     }
   }
   block_partial(acc, sm, partial);
