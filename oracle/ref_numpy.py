@@ -610,6 +610,25 @@ def smoothstep1(x, x_min, x_max):
     return (3.0 - 2.0 * t) * t ** 2
 
 
+# CCM89 coefficients (rubix/spectra/dust/extinction_models.py:124-177):
+# IR a, b = coeff * x^1.61; optical 7th-order polynomials in y = x - 1.82
+# (lowest order first, constant 1 / 0); NUV (c0, c1, c2, x_c, w) of
+# c0 + c1 x + c2 / ((x - x_c)^2 + w); far-NUV and FUV polynomial terms.
+_CCM89_IR = (0.574, -0.527, 1.61)
+_CCM89_OPT_A = [1.0, 0.17699, -0.50447, -0.02427, 0.72085, 0.01979, -0.77530, 0.32999]
+_CCM89_OPT_B = [0.0, 1.41338, 2.28305, 1.07233, -5.38434, -0.62251, 5.30260, -2.09002]
+_CCM89_NUV_A = (1.752, -0.316, -0.104, 4.67, 0.341)
+_CCM89_NUV_B = (-3.09, 1.825, 1.206, 4.62, 0.263)
+_CCM89_FNUV_A = (-0.04473, -0.009779)   # (y^2, y^3), y = x - 5.9
+_CCM89_FNUV_B = (0.2130, 0.1207)
+_CCM89_FUV_A = [-1.073, -0.628, 0.137, -0.070]   # y = x - 8, lowest order first
+_CCM89_FUV_B = [13.670, 4.257, -0.420, 0.374]
+
+
+def _poly(y, c):
+    return sum(ci * y ** i for i, ci in enumerate(c))
+
+
 def cardelli89(wave, Rv):
     """A(x)/A(V) of rubix/spectra/dust/extinction_models.py:104-178 (CCM89).
     The branches test the argument as a wavenumber x in 1/micron; the
@@ -623,22 +642,25 @@ def cardelli89(wave, Rv):
     nuv = (3.3 <= w) & (w <= 8.0)
     fnuv = (5.9 <= w) & (w <= 8)
     fuv = (8 < w) & (w <= 10)
-    a = np.where(ir, 0.574 * w ** 1.61, a)
-    b = np.where(ir, -0.527 * w ** 1.61, b)
+    ia, ib, ie = _CCM89_IR
+    a = np.where(ir, ia * w ** ie, a)
+    b = np.where(ir, ib * w ** ie, b)
     y = w - 1.82
-    a = np.where(opt, 1 + 0.17699 * y - 0.50447 * y ** 2 - 0.02427 * y ** 3 + 0.72085 * y ** 4
-                 + 0.01979 * y ** 5 - 0.77530 * y ** 6 + 0.32999 * y ** 7, a)
-    b = np.where(opt, 1.41338 * y + 2.28305 * y ** 2 + 1.07233 * y ** 3 - 5.38434 * y ** 4
-                 - 0.62251 * y ** 5 + 5.30260 * y ** 6 - 2.09002 * y ** 7, b)
+    a = np.where(opt, _poly(y, _CCM89_OPT_A), a)
+    b = np.where(opt, _poly(y, _CCM89_OPT_B), b)
     with np.errstate(divide="ignore", invalid="ignore"):
-        a = np.where(nuv, 1.752 - 0.316 * w - 0.104 / ((w - 4.67) ** 2 + 0.341), a)
-        b = np.where(nuv, -3.09 + 1.825 * w + 1.206 / ((w - 4.62) ** 2 + 0.263), b)
+        for c, arr in ((_CCM89_NUV_A, "a"), (_CCM89_NUV_B, "b")):
+            v = c[0] + c[1] * w + c[2] / ((w - c[3]) ** 2 + c[4])
+            if arr == "a":
+                a = np.where(nuv, v, a)
+            else:
+                b = np.where(nuv, v, b)
     y = w - 5.9
-    a = np.where(fnuv, a + (-0.04473 * y ** 2 - 0.009779 * y ** 3), a)
-    b = np.where(fnuv, b + (0.2130 * y ** 2 + 0.1207 * y ** 3), b)
+    a = np.where(fnuv, a + (_CCM89_FNUV_A[0] * y ** 2 + _CCM89_FNUV_A[1] * y ** 3), a)
+    b = np.where(fnuv, b + (_CCM89_FNUV_B[0] * y ** 2 + _CCM89_FNUV_B[1] * y ** 3), b)
     y = w - 8.0
-    a = np.where(fuv, -1.073 - 0.628 * y + 0.137 * y ** 2 - 0.070 * y ** 3, a)
-    b = np.where(fuv, 13.670 + 4.257 * y - 0.420 * y ** 2 + 0.374 * y ** 3, b)
+    a = np.where(fuv, _poly(y, _CCM89_FUV_A), a)
+    b = np.where(fuv, _poly(y, _CCM89_FUV_B), b)
     return a + b / Rv
 
 
@@ -648,6 +670,12 @@ _G23_OPT_A = [-0.35848, 0.7122, 0.08746, -0.05403, 0.00674,
               0.03893, 2.288, 0.243, 0.02965, 2.054, 0.179, 0.01747, 1.587, 0.243]
 _G23_OPT_B = [0.12354, -2.68335, 2.01901, -0.39299, 0.03355,
               0.18453, 2.288, 0.243, 0.19728, 2.054, 0.179, 0.1713, 1.587, 0.243]
+_G23_IR_B = (-1.01251, 1.0, -1.06099)        # power law (amplitude, x_0, alpha)
+_G23_UV_A = (0.81297, 0.2775, 1.06295, 0.11303, 4.60, 0.99)    # FM90 C1..C4, xo, gamma
+_G23_UV_B = (-2.97868, 1.89808, 3.10334, 0.65484, 4.60, 0.99)
+# wavelength bounds (micron): branches and the two smoothstep overlaps
+_G23_BRANCHES = {"ir": (1.0, 35.0), "opt": (0.3, 1.1), "uv": (0.09, 0.3),
+                 "optir": (0.9, 1.1), "uvopt": (0.3, 0.33)}
 
 
 def _g23_nirmir(w, p):
@@ -673,28 +701,29 @@ def gordon23(wave, Rv):
     w = np.asarray(wave, dtype=np.float64)
     a = np.zeros_like(w)
     b = np.zeros_like(w)
-    ir = (1.0 <= w) & (w < 35.0)
-    opt = (0.3 <= w) & (w < 1.1)
-    uv = (0.09 <= w) & (w <= 0.3)
-    optir = (w >= 0.9) & (w <= 1.1)
-    uvopt = (w >= 0.3) & (w <= 0.33)
+    B = _G23_BRANCHES
+    ir = (B["ir"][0] <= w) & (w < B["ir"][1])
+    opt = (B["opt"][0] <= w) & (w < B["opt"][1])
+    uv = (B["uv"][0] <= w) & (w <= B["uv"][1])
+    optir = (w >= B["optir"][0]) & (w <= B["optir"][1])
+    uvopt = (w >= B["uvopt"][0]) & (w <= B["uvopt"][1])
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
         ir_a = _g23_nirmir(w, _G23_IR_A)
-        ir_b = powerlaw1d(w, -1.01251, 1.0, -1.06099)
+        ir_b = powerlaw1d(w, *_G23_IR_B)
         op_a = _g23_optical(1 / w, _G23_OPT_A)
         op_b = _g23_optical(1 / w, _G23_OPT_B)
-        uv_a = fm90(1 / w, 0.81297, 0.2775, 1.06295, 0.11303, 4.60, 0.99)
-        uv_b = fm90(1 / w, -2.97868, 1.89808, 3.10334, 0.65484, 4.60, 0.99)
+        uv_a = fm90(1 / w, *_G23_UV_A)
+        uv_b = fm90(1 / w, *_G23_UV_B)
     a = np.where(ir, ir_a, a)
     b = np.where(ir, ir_b, b)
     a = np.where(opt, op_a, a)
     b = np.where(opt, op_b, b)
-    wt = smoothstep1(w, 0.9, 1.1)
+    wt = smoothstep1(w, *B["optir"])
     a = np.where(optir, (1.0 - wt) * op_a + wt * ir_a, a)
     b = np.where(optir, (1.0 - wt) * op_b + wt * ir_b, b)
     a = np.where(uv, uv_a, a)
     b = np.where(uv, uv_b, b)
-    wt = smoothstep1(w, 0.3, 0.33)
+    wt = smoothstep1(w, *B["uvopt"])
     a = np.where(uvopt, (1.0 - wt) * uv_a + wt * op_a, a)
     b = np.where(uvopt, (1.0 - wt) * uv_b + wt * op_b, b)
     return a + b * (1 / Rv - 1 / 3.1)

@@ -78,6 +78,66 @@ def calculate_wave_seq(wave_range, wave_res):
     return np.arange(wave_range[0], wave_range[1], wave_res, dtype=np.float32)
 
 
+def calculate_wave_edges(wave_seq, wave_res):
+    """rubix/telescope/utils.py:56-74: arange(first - res/2, last + res/2,
+    res) in float32 (bin edges around the wavelength sequence)."""
+    f = np.float32
+    start = f(f(wave_seq[0]) - f(wave_res / 2))
+    end = f(f(wave_seq[-1]) + f(wave_res / 2))
+    return np.arange(start, end, wave_res, dtype=f)
+
+
+def square_aperture(sbin):
+    """rubix/telescope/apertures.py:47-59: every spaxel (flattened sbin x sbin
+    ones)."""
+    return np.ones(int(sbin) * int(sbin), dtype=np.float32)
+
+
+def circular_aperture(sbin):
+    """rubix/telescope/apertures.py:62-81: spaxel (row r, column c) is in when
+    the distance of (c + 1, sbin - r) from (sbin/2 + 0.5, sbin/2 + 0.5) is
+    at most sbin/2, evaluated in float32; flattened row-major."""
+    f = np.float32
+    s = int(sbin)
+    centre = f(s / 2 + 0.5)
+    xx = np.tile(np.arange(1, s + 1, dtype=f), (s, 1)) - centre
+    yy = np.tile(np.arange(s, 0, -1, dtype=f), (s, 1)).T - centre
+    rr = np.sqrt(xx * xx + yy * yy)
+    return (rr <= f(s / 2)).astype(f).reshape(-1)
+
+
+def hexagonal_aperture(sbin):
+    """rubix/telescope/apertures.py:11-44: pixel (x, y), 1-based, with offsets
+    xx = x - c, yy = y - c from the centre c = sbin/2 + 0.5 is in the flat-
+    topped hexagon of half-width sbin/2 and half-height sbin*sqrt(3)/4 when
+    2 (sbin/4)(sbin sqrt3/4) - (sbin/4)|yy| - (sbin sqrt3/4)|xx| >= 0,
+    |xx| < sbin/2 and |yy| < sbin sqrt3/4, all in float32 as the reference
+    evaluates them; the mask is indexed [x - 1, y - 1] and flattened."""
+    f = np.float32
+    s = int(sbin)
+    c = s / 2 + 0.5
+    off = np.arange(1, s + 1, dtype=np.float64) - c          # exact half-integers
+    xx = np.abs(off.astype(f))[:, None]
+    yy = np.abs(off.astype(f))[None, :]
+    r3 = f(np.sqrt(f(3.0)))
+    h = f(f(f(s) * r3) / f(4.0))                             # sbin * sqrt(3) / 4
+    q = f(s / 4)
+    rr = f(f(2.0) * q * h) - q * yy - h * xx
+    inside = (rr >= 0) & (xx < f(s / 2)) & (yy < h)
+    return inside.astype(f).reshape(-1)
+
+
+APERTURES = {"square": square_aperture, "circular": circular_aperture,
+             "hexagonal": hexagonal_aperture}
+
+
+def get_aperture(kind, sbin):
+    """rubix/telescope/factory.py:102-110."""
+    if kind not in APERTURES:
+        raise ValueError(f"Unknown aperture type: {kind}")
+    return APERTURES[kind](sbin)
+
+
 @dataclass
 class Telescope:
     """The fields of rubix/telescope/base.py the hot path consumes."""
@@ -92,11 +152,14 @@ class Telescope:
     pixel_type: str
     aperture_type: str
     wave_seq: np.ndarray
+    aperture_region: Optional[np.ndarray] = None
+    wave_edges: Optional[np.ndarray] = None
 
 
 class TelescopeFactory:
     """rubix/telescope/factory.py:38-100: sbin = floor(fov / spatial_res),
-    wave_seq = arange(wave_range, wave_res)."""
+    the aperture mask of the instrument's aperture type, wave_seq =
+    arange(wave_range, wave_res) and its bin edges."""
 
     def __init__(self, telescopes_config_=None):
         self.telescopes_config = telescopes_config_ or telescopes_config
@@ -106,9 +169,13 @@ class TelescopeFactory:
             raise ValueError(f"Telescope {name} not found in config")
         c = self.telescopes_config[name]
         sbin = int(np.floor(c["fov"] / c["spatial_res"]))
+        aperture_type = c.get("aperture_type", "square")
+        aperture = get_aperture(aperture_type, sbin)
+        wave_seq = calculate_wave_seq(c["wave_range"], c["wave_res"])
         return Telescope(name=name, fov=c["fov"], spatial_res=c["spatial_res"],
                          wave_range=list(c["wave_range"]), wave_res=c["wave_res"],
                          lsf_fwhm=c.get("lsf_fwhm"), signal_to_noise=c.get("signal_to_noise"),
                          sbin=sbin, pixel_type=c.get("pixel_type", "square"),
-                         aperture_type=c.get("aperture_type", "square"),
-                         wave_seq=calculate_wave_seq(c["wave_range"], c["wave_res"]))
+                         aperture_type=aperture_type, wave_seq=wave_seq,
+                         aperture_region=aperture,
+                         wave_edges=calculate_wave_edges(wave_seq, c["wave_res"]))

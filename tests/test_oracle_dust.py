@@ -1,11 +1,125 @@
 """CPU: the dust restatement (oracle/ref_numpy.py dust section) against the
 reference's own known answers (tests/test_dust_classes.py,
-tests/test_dust_extinction.py), closed forms and the literal per-spaxel
-column-density algorithm on hand-checkable cases."""
+tests/test_dust_extinction.py), published curve values, closed forms and the
+literal per-spaxel column-density algorithm on hand-checkable cases.
+
+CCM89 is pinned by published values of the Cardelli, Clayton & Mathis (1989)
+curve at R(V) = 3.1 in every branch (FUV, far-NUV, NUV, optical, IR): the
+table the dust_extinction package (K. Gordon et al.; the implementation the
+reference follows, notebooks/dust_extinction.ipynb) checks its CCM89 model
+against.  Every coefficient of the restatement, perturbed by 0.1 %, fails
+that check.  G23 has no published table of curve values here: its
+coefficient tables are checked against the numbers of the reference's source
+(tests/golden/dust_coefficients.json, extracted from the file's syntax tree
+by tests/golden/make_dust_coefficients.py), and its structure against the
+normalisation it is built on (A(0.55 um)/A(V) = 1, E(0.44 - 0.55)/A(V) =
+1/R(V)), the agreement of neighbouring branches across their overlaps and
+the CCM89 curve.
+"""
+import copy
+import json
+import os
+
 import numpy as np
 import pytest
 
 from oracle import ref_numpy as R
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# x [1/micron] and A(x)/A(V) of CCM89 at R(V) = 3.1 (published, 8 decimals)
+CCM89_X = np.array([10.0, 9.0, 8.0, 7.0, 6.0, 5.0, 4.6, 4.0, 3.0, 2.78, 2.27, 1.82, 1.43, 1.11,
+                    0.8, 0.63, 0.46])
+CCM89_AXAV = np.array([5.23835484, 4.13406452, 3.33685933, 2.77962453, 2.52195399, 2.84252644,
+                       3.18598916, 2.31531711, 1.64254927, 1.56880904, 1.32257836, 1.0,
+                       0.75125994, 0.4780346, 0.28206957, 0.19200814, 0.11572348])
+
+
+def test_ccm89_published_values():
+    got = R.cardelli89(CCM89_X, 3.1)
+    np.testing.assert_allclose(got, CCM89_AXAV, rtol=0, atol=6e-9)
+
+
+_CCM89_TABLES = ["_CCM89_IR", "_CCM89_OPT_A", "_CCM89_OPT_B", "_CCM89_NUV_A", "_CCM89_NUV_B",
+                 "_CCM89_FNUV_A", "_CCM89_FNUV_B", "_CCM89_FUV_A", "_CCM89_FUV_B"]
+
+
+@pytest.mark.parametrize("table", _CCM89_TABLES)
+def test_ccm89_every_coefficient_is_pinned(table, monkeypatch):
+    """A 0.1 % change of any coefficient (the optical constants 1 and 0
+    excepted: those are the normalisation a(1.82) = 1, b(1.82) = 0 that the
+    x = 1.82 value pins) moves some published value by more than 1e-6."""
+    base = list(getattr(R, table))
+    for i, c in enumerate(base):
+        if c in (0.0, 1.0) and table.startswith("_CCM89_OPT"):
+            continue
+        bad = copy.copy(base)
+        bad[i] = c * 1.001
+        monkeypatch.setattr(R, table, type(getattr(R, table))(bad))
+        dev = np.abs(R.cardelli89(CCM89_X, 3.1) - CCM89_AXAV).max()
+        assert dev > 1e-6, (table, i, dev)
+    monkeypatch.setattr(R, table, type(getattr(R, table))(base))
+    assert np.abs(R.cardelli89(CCM89_X, 3.1) - CCM89_AXAV).max() < 6e-9
+
+
+def test_g23_tables_match_reference_source():
+    with open(os.path.join(GOLDEN, "dust_coefficients.json")) as f:
+        ref = json.load(f)
+    assert R._G23_IR_A == ref["ir_a"]
+    assert R._G23_OPT_A == ref["opt_a"]
+    assert R._G23_OPT_B == ref["opt_b"]
+    assert [list(R._G23_UV_A), list(R._G23_UV_B)] == ref["fm90_calls"]
+    assert ref["powerlaw_calls"] == [{"amplitude": R._G23_IR_B[0], "x_0": R._G23_IR_B[1],
+                                      "alpha": R._G23_IR_B[2]}]
+    assert list(R._G23_BRANCHES["ir"]) == ref["ir_mask"]
+    assert list(R._G23_BRANCHES["opt"]) == ref["opt_mask"]
+    assert list(R._G23_BRANCHES["uv"]) == ref["uv_mask"]
+    assert list(R._G23_BRANCHES["optir"]) == ref["optir_waves"]
+    assert list(R._G23_BRANCHES["uvopt"]) == ref["uvopt_waves"]
+
+
+def _g23_ab(w):
+    a = R.gordon23(np.asarray(w, dtype=np.float64), 3.1)
+    b = (R.gordon23(np.asarray(w, dtype=np.float64), 2.5) - a) / (1 / 2.5 - 1 / 3.1)
+    return a, b
+
+
+def test_g23_normalisation():
+    """G23 is fit as A(lambda)/A(V) = a + b (1/R(V) - 1/3.1) with monochromatic
+    V = 0.55 um and B = 0.44 um: a(V) = 1, b(V) = 0 and A(B)/A(V) - 1 =
+    E(B - V)/A(V) = 1/R(V) for every R(V), i.e. a(B) = 1 + 1/3.1, b(B) = 1,
+    to the accuracy of the fit."""
+    (av,), (bv,) = _g23_ab([0.55])
+    (ab,), (bb,) = _g23_ab([0.44])
+    assert abs(av - 1) < 0.02 and abs(bv) < 0.01
+    assert abs(ab - (1 + 1 / 3.1)) < 0.03 and abs(bb - 1) < 0.06
+    for rv in (2.5, 3.1, 4.0, 5.5):
+        ebv = R.gordon23(np.array([0.44]), rv)[0] - R.gordon23(np.array([0.55]), rv)[0]
+        assert abs(ebv * rv - 1) < 0.06, (rv, ebv * rv)
+
+
+def test_g23_branches_agree_across_overlaps():
+    """Where two branches are blended (0.9-1.1 um and 0.3-0.33 um) each one
+    alone is within a few per cent of the other: the blend joins two fits of
+    the same curve."""
+    w = np.linspace(0.9, 1.1, 21)
+    op = R._g23_optical(1 / w, R._G23_OPT_A)
+    ir = R._g23_nirmir(w, R._G23_IR_A)
+    assert np.abs(op / ir - 1).max() < 0.05
+    w = np.linspace(0.3, 0.33, 7)
+    uv = R.fm90(1 / w, *R._G23_UV_A)
+    op = R._g23_optical(1 / w, R._G23_OPT_A)
+    assert np.abs(uv / op - 1).max() < 0.04
+
+
+def test_g23_close_to_ccm89():
+    """Both are Milky Way average curves: between 0.17 and 3.2 um they agree
+    to 6 % in the UV/optical and drift to 16 % below CCM89 in the NIR."""
+    w = np.linspace(0.17, 3.2, 200)
+    r = R.gordon23(w, 3.1) / R.cardelli89(1 / w, 3.1)
+    assert r.min() > 0.83 and r.max() < 1.05
+    vis = (w > 0.3) & (w < 1.0)
+    assert np.abs(r[vis] - 1).max() < 0.06
 
 
 def test_generic_model_known_answers():
