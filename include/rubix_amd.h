@@ -207,13 +207,17 @@ typedef struct rbx_galaxy {
  * reduction chunk of the cube accumulation (16..65536, 0 = 256); waves =
  * wavefronts per workgroup of the per-star kernels (1, 2, 4, 8; 0 = 4);
  * persist = 0 for one workgroup per work item, k > 0 for persistent grids of
- * k workgroups per compute unit. */
+ * k workgroups per compute unit; flags: RBX_TUNE_TWO_PASS runs the forward
+ * as star_norm + per-tile accumulation (each (star, bin) evaluated twice)
+ * even where the one-pass fused kernel applies (the same values up to the
+ * grouping of the cube's float64 chunk sums). */
 typedef struct rbx_tuning {
   int32_t chunk;
   int32_t waves;
   int32_t persist;
-  int32_t reserved;
+  int32_t flags;
 } rbx_tuning;
+#define RBX_TUNE_TWO_PASS 1
 
 size_t rbx_batch_workspace_size(int64_t n, int32_t n_spaxel, int32_t n_target,
                                 const rbx_tuning* tune);

@@ -380,6 +380,45 @@ def apply_psf(datacube, psf_kernel):
     return convolve2d_same(np.asarray(datacube), np.asarray(psf_kernel))
 
 
+def gaussian_taps_1d(m, sigma, dtype=np.float64):
+    """The 1D factor g of gaussian_kernel_2d: exp(-x^2 / 2 sigma^2) over
+    x = arange(-(m-1)/2, (m-1)/2 + 1), normalised to sum 1.  In exact
+    arithmetic gaussian_kernel_2d(m, m, sigma) = outer(g, g), since
+    exp(-(x^2 + y^2)/2s^2) = exp(-x^2/2s^2) exp(-y^2/2s^2) and the sum of the
+    outer product is the product of the sums."""
+    x = np.arange(-((m - 1) / 2), ((m - 1) / 2) + 1).astype(dtype)
+    v = np.exp(-(x ** 2) / (dtype(2) * dtype(sigma) ** 2))
+    return (v / v.sum()).astype(dtype)
+
+
+def apply_psf_separable(datacube, g):
+    """apply_psf with the kernel outer(g, g): the 'same' zero-padded
+    convolution along the y axis, then along the x axis (a separable
+    restatement of convolve2d(mode="same") for outer-product kernels, for
+    cubes where the direct sum over m^2 taps is too slow)."""
+    cube = np.asarray(datacube)
+    g = np.asarray(g)
+    m = len(g)
+    s = (m - 1) // 2
+
+    def along(a, axis):
+        n = a.shape[axis]
+        out = np.zeros_like(a)
+        for t in range(m):
+            d = s - t  # out[i] += g[t] * a[i + d]
+            lo, hi = max(0, -d), min(n, n - d)
+            if lo >= hi:
+                continue
+            dst = [slice(None)] * a.ndim
+            src = [slice(None)] * a.ndim
+            dst[axis] = slice(lo, hi)
+            src[axis] = slice(lo + d, hi + d)
+            out[tuple(dst)] += g[t] * a[tuple(src)]
+        return out
+
+    return along(along(cube, 0), 1)
+
+
 def gaussian1d(x, sigma):
     """rubix/telescope/lsf/lsf.py:12-15."""
     x = np.asarray(x)
@@ -818,7 +857,7 @@ def apply_spaxel_extinction(spectra, wave_target, av, model, Rv):
 # ---------------------------------------------------------------- path
 
 def plain_forward(stars, grid, geom, dtype=np.float32, chunk=4096,
-                  return_intermediates=False, ref_knots=True):
+                  return_intermediates=False, ref_knots=True, psf="direct", raw_only=False):
     """The calc_ifu step chain (rubix/config/pipeline_config.yml:1-60) from
     spaxel_assignment to convolve_lsf, at D = 1:
 
@@ -838,6 +877,10 @@ def plain_forward(stars, grid, geom, dtype=np.float32, chunk=4096,
     interpolation whose rounding moves interpolation weights by ~1e-4, so this
     is the float64 evaluation of the reference's own float32 algorithm;
     ``ref_knots=False`` gives exact arithmetic throughout.
+    ``psf="separable"`` applies the PSF as two 1D passes (apply_psf_separable),
+    equal to the direct 2D sum in exact arithmetic, for large cubes.
+    ``raw_only`` returns the datacube before the PSF (calculate_datacube's
+    output).
     """
     edges = np.asarray(geom["edges"], dtype=np.float32)
     nb = len(edges) - 1
@@ -858,9 +901,14 @@ def plain_forward(stars, grid, geom, dtype=np.float32, chunk=4096,
         part = calculate_cube(res, ids[s:e], nb).reshape(nb * nb, -1)
         cube += part
     cube = cube.reshape(nb, nb, -1)
+    if raw_only:
+        return cube
     m = geom["psf_size"]
-    K = gaussian_kernel_2d(m, m, geom["psf_sigma"], dtype=dtype)
-    c_psf = apply_psf(cube, K)
+    if psf == "separable":
+        c_psf = apply_psf_separable(cube, gaussian_taps_1d(m, geom["psf_sigma"], dtype=dtype))
+    else:
+        K = gaussian_kernel_2d(m, m, geom["psf_sigma"], dtype=dtype)
+        c_psf = apply_psf(cube, K)
     c_lsf = apply_lsf_kernel(c_psf, lsf_kernel(geom["lsf_sigma"], geom["wave_res"], dtype=dtype))
     if return_intermediates:
         return dict(ids=ids, cube=cube, cube_psf=c_psf, cube_lsf=c_lsf)

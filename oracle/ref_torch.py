@@ -129,6 +129,28 @@ def apply_psf(cube, K):
     return out
 
 
+def gaussian_taps_1d(m, sigma):
+    """1D factor g of gaussian_kernel_2d (outer(g, g) in exact arithmetic)."""
+    x = torch.arange(-((m - 1) / 2), ((m - 1) / 2) + 1, dtype=sigma.dtype)
+    v = torch.exp(-(x ** 2) / (2 * sigma ** 2))
+    return v / v.sum()
+
+
+def apply_psf_separable(cube, g):
+    """apply_psf with the kernel outer(g, g) as two 1D 'same' zero-padded
+    convolutions (y axis, then x axis): conv1d with the flipped taps over
+    each line, padding (m - 1)/2 on both sides (odd m)."""
+    ny, nx, nl = cube.shape
+    m = g.numel()
+    k = torch.flip(g, dims=(0,))[None, None, :]
+    # along y: lines are (x, lambda) columns
+    a = cube.permute(1, 2, 0).reshape(-1, 1, ny)
+    a = F.conv1d(a, k, padding=(m - 1) // 2).reshape(nx, nl, ny).permute(2, 0, 1)
+    b = a.permute(0, 2, 1).reshape(-1, 1, nx)
+    b = F.conv1d(b, k, padding=(m - 1) // 2).reshape(ny, nl, nx).permute(0, 2, 1)
+    return b
+
+
 def lsf_kernel(sigma, wave_res, factor=12):
     """rubix/telescope/lsf/lsf.py:12-26 with a tensor sigma (Angstrom)."""
     import numpy as np
@@ -194,13 +216,17 @@ def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots=True):
 
 
 def full_forward(mass, age, met, vel_los, ids, sigma_psf, sigma_lsf, grid, geom,
-                 dtype=torch.float64, ref_knots=True):
+                 dtype=torch.float64, ref_knots=True, psf="direct"):
     """Whole path with tensors for every differentiable input; returns
-    (raw cube, cube after PSF, cube after LSF)."""
+    (raw cube, cube after PSF, cube after LSF).  psf="separable": the PSF as
+    two 1D passes (equal in exact arithmetic), for large cubes."""
     cube = star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots)
     m = geom["psf_size"]
-    K = gaussian_kernel_2d(m, m, sigma_psf)
-    c1 = apply_psf(cube, K)
+    if psf == "separable":
+        c1 = apply_psf_separable(cube, gaussian_taps_1d(m, sigma_psf))
+    else:
+        K = gaussian_kernel_2d(m, m, sigma_psf)
+        c1 = apply_psf(cube, K)
     k = lsf_kernel(sigma_lsf, geom["wave_res"])
     c2 = apply_lsf(c1, k)
     return cube, c1, c2

@@ -46,7 +46,7 @@ class RbxGalaxy(ctypes.Structure):
 
 class RbxTuning(ctypes.Structure):
     _fields_ = [("chunk", c_int32), ("waves", c_int32), ("persist", c_int32),
-                ("reserved", c_int32)]
+                ("flags", c_int32)]
 
 
 class RbxRotation(ctypes.Structure):

@@ -667,14 +667,14 @@ __global__ void __launch_bounds__(64) cube_accum_kernel(
   }
 }
 
-// Bins per lane and slab of the fused pass (at most 8 x 64 per slab, so
-// targets up to 4096 bins take it).
-constexpr int kLaneBins = 8;
+// Bins per lane of the fused pass: NB = 4 for targets of up to 4096 bins
+// (8 slabs x 2 wavefronts x 64 lanes x 4), up to 8 for 8192.
+constexpr int kMaxFusedBins = 8;
 
 // Normalisation and accumulation in one pass for ascending targets: item q
 // is spaxel chunk q, one workgroup of 16 wavefronts, two per lambda slab
-// (wavefront 2b + h takes the bins j0 + 256 h + lane + 64 t, t < 4, of slab
-// b).  Per star of the chunk each wavefront stages the knot window of its
+// (wavefront 2b + h takes the bins j0 + 64 NB h + lane + 64 t, t < NB, of
+// slab b; NB = 4 up to 4096 target bins).  Per star of the chunk each wavefront stages the knot window of its
 // own 256 bins in its LDS (no other wavefront reads it) and evaluates P_j on
 // them once (the per-bin arithmetic of star_norm_kernel and
 // cube_accum_kernel), then posts its share of Nn = sum_j P_j dtarget_j;
@@ -704,15 +704,23 @@ __device__ inline float sum16_row0(float v) {
   v = plus_dpp<0x128>(v);  // row_ror:8
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
-constexpr int kFusedBins = kLaneBins / 2;  // bins per lane
-constexpr int kHalfWindow = 320;           // knots per staged half-slab window
+// Knots of a wavefront's staged window: 80 per bin of a lane (the 64 NB
+// bins of a wavefront span ~64 NB x 1.25 A, ~57 NB knots of a 1.4 A grid,
+// plus the margins); wider windows take the global search.
+template <int NB>
+constexpr int fused_window() { return 80 * NB; }
 
-template <bool kDust>
-__global__ void __launch_bounds__(64 * kFusedWaves) __attribute__((amdgpu_waves_per_eu(8)))
+// Up to 4 bins per lane the kernel holds to 64 VGPRs (8 wavefronts per
+// SIMD); more bins per lane take 128 (4 per SIMD).
+template <bool kDust, int NB>
+__global__ void __launch_bounds__(64 * kFusedWaves)
+__attribute__((amdgpu_waves_per_eu(NB <= 4 ? 8 : 4)))
 cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
                   const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr,
                   int slab_bins, int64_t n_items, const float* __restrict__ axav,
                   double* __restrict__ partial) {
+  constexpr int kFusedBins = NB;
+  constexpr int kHalfWindow = fused_window<NB>();
   __shared__ float lam_s[kFusedWaves][kHalfWindow];
   __shared__ float F_s[kFusedWaves][kHalfWindow];
   __shared__ float nn_s[2][kFusedWaves];
@@ -1384,10 +1392,12 @@ int tune_of(const rbx_tuning* t, Tune* out) {
   d.chunk = 256;
   d.waves = 4;
   d.persist = 0;
+  d.flags = 0;
   if (t) {
     if (t->chunk != 0) d.chunk = t->chunk;
     if (t->waves != 0) d.waves = t->waves;
     d.persist = t->persist;
+    d.flags = t->flags;
   }
   if (d.chunk < 16 || d.chunk > 65536)
     return fail(RBX_EINVAL, "rbx_tuning: chunk must be in [16, 65536] (0 = default)");
@@ -1395,6 +1405,8 @@ int tune_of(const rbx_tuning* t, Tune* out) {
     return fail(RBX_EINVAL, "rbx_tuning: waves must be 1, 2, 4 or 8 (0 = default)");
   if (d.persist < 0 || d.persist > 64)
     return fail(RBX_EINVAL, "rbx_tuning: persist must be in [0, 64]");
+  if (d.flags & ~RBX_TUNE_TWO_PASS)
+    return fail(RBX_EINVAL, "rbx_tuning: unknown flags");
   *out = d;
   return RBX_OK;
 }
@@ -1508,6 +1520,32 @@ size_t cube_workspace_impl(int64_t n, int32_t n_keys, int32_t nt, const Tune& t)
   return so.s.used;
 }
 
+struct FusedLaunch {
+  int64_t grid;
+  hipStream_t s;
+  SspView g;
+  rbx_target tg;
+  StarState* state;
+  const ChunkDesc* desc;
+  const int32_t* n_chunks;
+  int slab_bins;
+  int64_t items;
+  const float* axav;
+  double* partial;
+};
+
+template <int NB>
+void launch_fused(const FusedLaunch& f) {
+  if (f.axav)
+    hipLaunchKernelGGL((cube_fused_kernel<true, NB>), dim3(f.grid), dim3(64 * kFusedWaves), 0,
+                       f.s, f.g, f.tg, f.state, f.desc, f.n_chunks, f.slab_bins, f.items, f.axav,
+                       f.partial);
+  else
+    hipLaunchKernelGGL((cube_fused_kernel<false, NB>), dim3(f.grid), dim3(64 * kFusedWaves), 0,
+                       f.s, f.g, f.tg, f.state, f.desc, f.n_chunks, f.slab_bins, f.items,
+                       nullptr, f.partial);
+}
+
 int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars* stars,
                   const rbx_extinction* ext, const Tune& t, float* cube, void* workspace,
                   size_t workspace_bytes, hipStream_t s, const char* who) {
@@ -1535,10 +1573,12 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   if (rc) return rc;
   const int n_tiles = (nt + kTile - 1) / kTile;
   const int tps = (n_tiles + kSlabs - 1) / kSlabs;
-  // ascending targets with at most kLaneBins bins per lane and slab: one
-  // fused pass (cube_fused_kernel); otherwise star_norm + star_sf, then the
-  // per-tile accumulation
-  const bool fused = target->ascending && tps <= kLaneBins;
+  // ascending targets with at most kMaxFusedBins bins per lane: one fused
+  // pass (cube_fused_kernel, NB bins per lane); otherwise (or when the
+  // tuning asks for it) star_norm + star_sf, then the per-tile accumulation
+  const int nb_lane = (tps + 1) / 2 < 4 ? 4 : (tps + 1) / 2;
+  const bool fused = target->ascending && nb_lane <= kMaxFusedBins &&
+                     !(t.flags & RBX_TUNE_TWO_PASS);
   if (n > 0) {
     hipLaunchKernelGGL(star_prep_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, g, *target,
                        *stars, L.perm, L.skey, ext ? ext->av : nullptr, L.state, L.gradw);
@@ -1562,14 +1602,15 @@ int cube_fwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   if (n > 0 && fused) {
     const int64_t items = L.max_chunks;
     const int64_t grid = item_grid(items, t.persist, 64 * kFusedWaves);
-    if (axav)
-      hipLaunchKernelGGL(cube_fused_kernel<true>, dim3(grid), dim3(64 * kFusedWaves), 0, s, g,
-                         *target, L.state, L.desc, L.chunk_off + n_band_keys, tps * kTile, items, axav,
-                         L.partial);
-    else
-      hipLaunchKernelGGL(cube_fused_kernel<false>, dim3(grid), dim3(64 * kFusedWaves), 0, s, g,
-                         *target, L.state, L.desc, L.chunk_off + n_band_keys, tps * kTile, items, nullptr,
-                         L.partial);
+    const FusedLaunch f{grid, s, g, *target, L.state, L.desc, L.chunk_off + n_band_keys,
+                        tps * kTile, items, axav, L.partial};
+    switch (nb_lane) {
+      case 4: launch_fused<4>(f); break;
+      case 5: launch_fused<5>(f); break;
+      case 6: launch_fused<6>(f); break;
+      case 7: launch_fused<7>(f); break;
+      default: launch_fused<8>(f);
+    }
     if ((rc = check_launch(who))) return rc;
   } else if (n > 0) {
     const int64_t items = int64_t(8) * tpx * L.max_chunks;

@@ -11,6 +11,7 @@ struct Tune {
   int chunk;    // stars per reduction chunk of cube_accum
   int waves;    // wavefronts per workgroup of star_norm / star_bwd
   int persist;  // 0 = one workgroup per work item, k = k workgroups per CU
+  int flags;    // RBX_TUNE_TWO_PASS: no one-pass fused forward
 };
 
 int tune_of(const rbx_tuning* t, Tune* out);

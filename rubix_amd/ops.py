@@ -399,18 +399,23 @@ def conv_axis_dot(a, b, taps, n_outer, n_axis, n_inner, start):
 
 # ---------------------------------------------------------------- batches
 
+TUNE_TWO_PASS = 1  # rbx_tuning.flags: star_norm + per-tile accumulation forward
+
+
 @dataclass(frozen=True)
 class Tuning:
     """Launch tuning of the fused kernels (rbx_tuning): stars per reduction
     chunk, wavefronts per workgroup of the per-star kernels, persistent
-    workgroups per CU (0 = one workgroup per work item)."""
+    workgroups per CU (0 = one workgroup per work item), flags
+    (TUNE_TWO_PASS)."""
     chunk: int = 256
     waves: int = 4
     persist: int = 0
+    flags: int = 0
 
     @property
     def struct(self):
-        return RbxTuning(self.chunk, self.waves, self.persist, 0)
+        return RbxTuning(self.chunk, self.waves, self.persist, self.flags)
 
 
 class BatchLayout:

@@ -94,3 +94,7 @@ def _synthetic(n_met, n_age, n_wave, wave_range):
 
 register_template("synthetic_6x20x1000", _synthetic(6, 20, 1000, (3000.0, 9000.0)))
 register_template("synthetic_12x100x5000", _synthetic(12, 100, 5000, (3000.0, 10000.0)))
+# near-infrared (8000-20000 A, 4 A spacing) and ultraviolet-to-5.6 micron
+# (2500-56000 A, ~13 A spacing) grids for the instruments beyond the optical
+register_template("synthetic_6x20x3000_nir", _synthetic(6, 20, 3000, (8000.0, 20000.0)))
+register_template("synthetic_6x20x4000_wide", _synthetic(6, 20, 4000, (2500.0, 56000.0)))

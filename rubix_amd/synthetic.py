@@ -235,7 +235,56 @@ def config4_sizes(n_gal=64, seed=3):
     return ns, nbs
 
 
-SSP_TEMPLATE_NAMES = {(6, 20, 1000): "synthetic_6x20x1000", (12, 100, 5000): "synthetic_12x100x5000"}
+SSP_TEMPLATE_NAMES = {(6, 20, 1000): "synthetic_6x20x1000", (12, 100, 5000): "synthetic_12x100x5000",
+                      (6, 20, 3000): "synthetic_6x20x3000_nir",
+                      (6, 20, 4000): "synthetic_6x20x4000_wide"}
+SSP_TEMPLATE_RANGES = {"synthetic_6x20x1000": (6, 20, 1000, (3000.0, 9000.0)),
+                       "synthetic_12x100x5000": (12, 100, 5000, (3000.0, 10000.0)),
+                       "synthetic_6x20x3000_nir": (6, 20, 3000, (8000.0, 20000.0)),
+                       "synthetic_6x20x4000_wide": (6, 20, 4000, (2500.0, 56000.0))}
+
+
+def instrument_workload(name, template, n_stars, z=0.05, psf_sigma=1.0, psf_size=5, seed=0,
+                        h_r=3.0, telescopes=None):
+    """Stars, grid and geometry of a named instrument of the telescope table:
+    nb = floor(fov / spatial_res), spaxel edges from the float32 arange of
+    the angular aperture at redshift z (the reference's construction,
+    rubix/telescope/utils.py:11-37), target wavelengths = the instrument's
+    float32 wave_seq, PSF sigma in pixels on psf_size^2 taps, LSF sigma =
+    lsf_fwhm / (2 sqrt(2 ln 2)) Angstrom at the instrument's wave_res.  The
+    grid is the registered synthetic template `template`."""
+    from .telescope import TelescopeFactory, calculate_spatial_bin_edges
+
+    tel = TelescopeFactory(telescopes).create_telescope(name)
+    edges, _ = calculate_spatial_bin_edges(tel.fov, tel.sbin, z)
+    nm, na, nw, rng = SSP_TEMPLATE_RANGES[template]
+    grid = make_ssp_grid(nm, na, nw, rng, seed=1)
+    lsf_sigma = float(tel.lsf_fwhm) / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+    geom = dict(edges=np.asarray(edges, np.float32), nb=int(tel.sbin),
+                wave_target=np.asarray(tel.wave_seq, np.float32), z=float(z),
+                psf_sigma=float(psf_sigma), psf_size=int(psf_size), lsf_sigma=lsf_sigma,
+                wave_res=float(tel.wave_res), fov_arcsec=float(tel.fov),
+                aperture_kpc=float(edges[-1] - edges[0]))
+    stars = make_stars(n_stars, seed=seed, h_r=h_r)
+    return Workload(name, stars, grid, geom, meta={"template": template})
+
+
+def instrument_config(w: Workload, device="cuda"):
+    """Step-API config naming the instrument itself (no explicit edges or
+    wavelengths: the steps derive them from the telescope table as the
+    reference does)."""
+    g = w.geom
+    return {
+        "device": device,
+        "galaxy": {"dist_z": g["z"], "rotation": {"type": "face-on"}},
+        "data": {"args": {"particle_type": ["stars"]}},
+        "telescope": {"name": w.name,
+                      "psf": {"name": "gaussian", "size": g["psf_size"], "sigma": g["psf_sigma"]},
+                      "lsf": {"sigma": g["lsf_sigma"]},
+                      "noise": {"signal_to_noise": 1, "noise_distribution": "normal"}},
+        "ssp": {"template": {"name": w.meta["template"]}, "method": "linear"},
+        "cosmology": {"name": "PLANCK15"},
+    }
 
 
 def step_config(w: Workload, device="cuda"):

@@ -82,8 +82,10 @@ def _gpu_grads(w, tgt, sig, cotangent=None):
 
 @pytest.mark.parametrize("cfg", ["cfg1", "cfg2"])
 def test_gradients_match_oracle(cfg):
+    """cfg1: the differentiation workload at its own size (10 000 stars, 25 x 25
+    x 1000 cube); cfg2: a 150-star sample of the large grid and cube."""
     if cfg == "cfg1":
-        w = with_kernels(S.config1(n_stars=600))
+        w = with_kernels(S.config1(n_stars=10_000))
     else:
         w = with_kernels(S.config2(n_stars=150))
     tgt, sig = _target(w)
@@ -106,7 +108,7 @@ def test_adjoint_with_oracle_cotangent(cfg):
     """The adjoint alone: the oracle's dL/dC (chi^2 residual of the float64
     forward) is fed to the GPU backward, so forward rounding amplified by the
     residual (which is 1/30 of the signal) does not enter."""
-    w = S.config1(n_stars=600) if cfg == "cfg1" else S.config2(n_stars=150)
+    w = S.config1(n_stars=10_000) if cfg == "cfg1" else S.config2(n_stars=150)
     tgt, sig = _target(w)
     clean = R.plain_forward(w.stars, w.grid, w.geom, dtype=np.float64)
     cot = 2.0 * (clean - tgt) / sig ** 2

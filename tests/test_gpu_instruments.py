@@ -1,0 +1,146 @@
+"""GPU: the step API for every instrument of the reference's telescope table
+(rubix/telescope/telescopes.yaml:1-150), configs naming the instrument only
+(edges and wavelengths derived from the table as the reference does).
+
+* Three instrument shapes against the float64 oracle, the cube after
+  PSF + LSF and the per-star gradients (mass, age, metallicity, v_los) of
+  <G, cube> for a seeded dL/dcube G, norm-wise <= 1e-5:
+  MaNGA_32 (64^2 spaxels, 6474 bins: more than 4096, with the dense
+  12 x 100 x 5000 grid), HECTOR (300^2 = 90 000 spaxels, 1369 bins) and
+  NIRSpec_G140H_F100LP (30^2, 3756 bins with a near-infrared grid).  The
+  oracle applies the PSF as two 1D passes (tests/test_oracle.py).
+* MaNGA_32's forward also through the two-kernel path (star_norm + per-tile
+  accumulation, forced by the tuning flag) with a grid dense enough that some
+  slab windows exceed the LDS window and take the global search.
+* calc_ifu_fused, the whole chain with rotation, aperture filter and noise,
+  runs for all 15 instruments and gives a finite (sbin, sbin, n_wave) cube.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_numpy as R, ref_torch as RT
+from rubix_amd import ops, synthetic as S
+from rubix_amd.config import telescopes_config
+from rubix_amd.core.data import stars_from_arrays
+from rubix_amd.core.pipeline import RubixPipeline
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+TOL = 1e-5
+
+SHAPES = [
+    ("MaNGA_32", "synthetic_12x100x5000", 2000, dict(psf_sigma=2.0, psf_size=11, h_r=3.0)),
+    ("HECTOR", "synthetic_6x20x1000", 3000, dict(psf_sigma=1.5, psf_size=7, h_r=3.0)),
+    ("NIRSpec_G140H_F100LP", "synthetic_6x20x3000_nir", 2000,
+     dict(psf_sigma=0.8, psf_size=5, h_r=1.0)),
+]
+
+
+def _oracle_grads(w, G):
+    st = w.stars
+    f64 = torch.float64
+    T = {k: torch.tensor(st[k], dtype=f64, requires_grad=True) for k in ("mass", "age", "metallicity")}
+    vl = torch.tensor(st["velocity"][:, 2], dtype=f64, requires_grad=True)
+    ids = torch.tensor(R.square_spaxel_assignment(st["coords"], w.geom["edges"]).astype(np.int64))
+    _, _, c2 = RT.full_forward(T["mass"], T["age"], T["metallicity"], vl, ids,
+                               torch.tensor(w.geom["psf_sigma"], dtype=f64),
+                               torch.tensor(w.geom["lsf_sigma"], dtype=f64), w.grid, w.geom, f64,
+                               psf="separable")
+    c2.backward(torch.tensor(G, dtype=f64))
+    return c2.detach().numpy(), [T["mass"].grad.numpy(), T["age"].grad.numpy(),
+                                 T["metallicity"].grad.numpy(), vl.grad.numpy()]
+
+
+@pytest.mark.parametrize("name,template,n,kw", SHAPES, ids=[s[0] for s in SHAPES])
+def test_instrument_shape_matches_oracle(name, template, n, kw):
+    w = S.instrument_workload(name, template, n, **kw)
+    nb, nt = w.geom["nb"], len(w.geom["wave_target"])
+    cfg = S.instrument_config(w)
+    st = w.stars
+    rd = stars_from_arrays(st["coords"], st["velocity"], st["mass"], st["age"], st["metallicity"],
+                           requires_grad=True)
+    leaves = (rd.stars.mass, rd.stars.age, rd.stars.metallicity, rd.stars.velocity)
+    out = RubixPipeline(cfg, "calc_ifu_fused_aligned").run(rd)
+    cube = out.stars.datacube
+    assert tuple(cube.shape) == (nb, nb, nt)
+    G = np.random.default_rng(3).normal(size=(nb, nb, nt))
+    ref, g_o = _oracle_grads(w, G)
+    e = rel_err(cube.detach().cpu().numpy(), ref)
+    print(f"{name} {nb}x{nb}x{nt}: cube rel64={e:.3e}")
+    assert e <= TOL
+    cube.backward(torch.as_tensor(G, dtype=torch.float32, device=DEV))
+    got = [leaves[0].grad, leaves[1].grad, leaves[2].grad, leaves[3].grad.reshape(-1, 3)[:, 2]]
+    for nm, a, b in zip(("mass", "age", "metallicity", "v_los"), got, g_o):
+        e = rel_err(a.reshape(-1).cpu().numpy(), b)
+        print(f"{name}: d/d{nm} rel={e:.3e}")
+        assert e <= TOL, (nm, e)
+
+
+@pytest.mark.parametrize("template", ["synthetic_12x100x5000", "dense"])
+def test_long_axis_two_kernel_path(template):
+    """MaNGA_32's 6474 bins through star_norm + star_sf + the per-tile
+    accumulation (tuning flag RBX_TUNE_TWO_PASS), against the oracle's raw
+    cube, and its adjoint against the gradient oracle.  The dense grid (0.58 A
+    knots, over 1400 knots per 810-bin slab) exceeds the 640-knot LDS window,
+    so those slabs take the global search."""
+    kw = dict(psf_sigma=2.0, psf_size=11)
+    if template == "dense":
+        w = S.instrument_workload("MaNGA_32", "synthetic_12x100x5000", 1500, **kw)
+        w.grid = S.make_ssp_grid(4, 6, 12000, (3000.0, 10000.0), seed=1)
+    else:
+        w = S.instrument_workload("MaNGA_32", template, 1500, **kw)
+    g = w.grid
+    nb, nt = w.geom["nb"], len(w.geom["wave_target"])
+    ssp = ops.PreparedSSP(g["flux"], g["ax_met"], g["ax_age"], g["wave"], w.geom["z"],
+                          g["log_axes"], DEV)
+    tg = ops.TargetGrid(w.geom["wave_target"], DEV)
+    t = {k: torch.as_tensor(v, device=DEV) for k, v in w.stars.items()}
+    ids = ops.spaxel_assign(t["coords"], torch.as_tensor(w.geom["edges"], device=DEV))
+    layout = ops.BatchLayout([w.n_stars], [nb], DEV)
+    ref = R.plain_forward(w.stars, g, w.geom, dtype=np.float64, raw_only=True)
+    cubes = {}
+    for label, tune in (("fused", ops.Tuning()), ("two-pass", ops.Tuning(flags=ops.TUNE_TWO_PASS))):
+        cube, ctx = ops.batch_forward(ssp, tg, layout, t["mass"], t["age"], t["metallicity"],
+                                      t["velocity"], ids, tune=tune)
+        e = rel_err(cube.cpu().numpy().reshape(nb, nb, nt), ref)
+        print(f"MaNGA_32 {template} {label}: raw cube rel64={e:.3e}")
+        assert e <= TOL, (label, e)
+        cubes[label] = (cube, ctx, tune)
+    gen = torch.Generator(device=DEV).manual_seed(9)
+    Gc = torch.randn(nb * nb, nt, generator=gen, device=DEV)
+    st = w.stars
+    f64 = torch.float64
+    T = {k: torch.tensor(st[k], dtype=f64, requires_grad=True) for k in ("mass", "age", "metallicity")}
+    vl = torch.tensor(st["velocity"][:, 2], dtype=f64, requires_grad=True)
+    raw = RT.star_cube(T["mass"], T["age"], T["metallicity"], vl,
+                       torch.tensor(R.square_spaxel_assignment(st["coords"], w.geom["edges"])
+                                    .astype(np.int64)), g, w.geom, f64)
+    raw.backward(torch.tensor(Gc.cpu().numpy().reshape(nb, nb, nt), dtype=f64))
+    g_o = [T["mass"].grad.numpy(), T["age"].grad.numpy(), T["metallicity"].grad.numpy(),
+           vl.grad.numpy()]
+    for label, (cube, ctx, tune) in cubes.items():
+        grads = ops.batch_backward(ssp, tg, layout, ctx, Gc, tune=tune)
+        for nm, a, b in zip(("mass", "age", "metallicity", "v_los"), grads, g_o):
+            e = rel_err(a.cpu().numpy(), b)
+            print(f"MaNGA_32 {template} {label}: d/d{nm} rel={e:.3e}")
+            assert e <= TOL, (label, nm, e)
+
+
+@pytest.mark.parametrize("name", sorted(telescopes_config))
+def test_calc_ifu_fused_runs_for_every_instrument(name):
+    tel = telescopes_config[name]
+    template = ("synthetic_12x100x5000" if tel["wave_range"][1] <= 10400 else
+                "synthetic_6x20x4000_wide")
+    w = S.instrument_workload(name, template, 2000, h_r=1.0 if tel["fov"] < 10 else 3.0)
+    cfg = S.instrument_config(w)
+    st = w.stars
+    rd = stars_from_arrays(st["coords"], st["velocity"], st["mass"], st["age"], st["metallicity"],
+                           halfmassrad_stars=2.0)
+    out = RubixPipeline(cfg, "calc_ifu_fused").run(rd)
+    cube = out.stars.datacube
+    nb, nt = w.geom["nb"], len(w.geom["wave_target"])
+    assert tuple(cube.shape) == (nb, nb, nt), (name, tuple(cube.shape))
+    assert bool(torch.isfinite(cube).all()), name
+    assert float(cube.abs().sum()) > 0, name

@@ -262,3 +262,33 @@ def test_lsf_preserves_shape_and_integral():
     assert out.shape == cube.shape
     assert np.all(out[:, :, 5] != 1)
     np.testing.assert_allclose(out.sum(axis=2), 1.0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("ny,nx,nl,m,sig", [(25, 25, 7, 11, 1.2), (9, 13, 3, 15, 4.0),
+                                             (3, 4, 2, 5, 0.7), (40, 40, 5, 1, 1.0),
+                                             (150, 20, 2, 11, 3.0)])
+def test_separable_psf_equals_direct(ny, nx, nl, m, sig):
+    """The separable PSF restatements (numpy and torch, used for the largest
+    cubes) equal the direct convolve2d 'same' sum, and the torch one has the
+    same sigma gradient."""
+    import torch
+
+    from oracle import ref_torch as RT
+
+    rng = np.random.default_rng(ny * nx + m)
+    c = rng.normal(size=(ny, nx, nl))
+    d = R.apply_psf(c, R.gaussian_kernel_2d(m, m, sig, dtype=np.float64))
+    np.testing.assert_allclose(np.outer(R.gaussian_taps_1d(m, sig), R.gaussian_taps_1d(m, sig)),
+                               R.gaussian_kernel_2d(m, m, sig, dtype=np.float64), rtol=1e-14, atol=0)
+    s = R.apply_psf_separable(c, R.gaussian_taps_1d(m, sig))
+    assert np.abs(s - d).max() <= 1e-14 * np.abs(d).max()
+    ct = torch.tensor(c)
+    g = rng.normal(size=c.shape)
+    sa = torch.tensor(sig, dtype=torch.float64, requires_grad=True)
+    sb = torch.tensor(sig, dtype=torch.float64, requires_grad=True)
+    a = RT.apply_psf(ct, RT.gaussian_kernel_2d(m, m, sa))
+    b = RT.apply_psf_separable(ct, RT.gaussian_taps_1d(m, sb))
+    assert (a - b).abs().max().item() <= 1e-14 * a.abs().max().item()
+    (a * torch.tensor(g)).sum().backward()
+    (b * torch.tensor(g)).sum().backward()
+    assert abs(float(sa.grad) - float(sb.grad)) <= 1e-12 * max(abs(float(sa.grad)), 1e-30)
