@@ -8,6 +8,18 @@ w.r.t. every star's mass, age, metallicity and v_los and both sigmas).
 Inputs are resident in HBM before timing starts.
 
 Prints one JSON line.  Metric: star*lambda-bins/s = N * n_target / t(fwd+adj).
+
+Roofline of the dominant launch: the per-star kernels issue one VALU wave-
+instruction per 4 cycles per SIMD at best and are bound by that issue rate
+(their HBM traffic is a few per cent of the 8 TB/s peak), so `bound` is
+"valu": achieved = VALU wave-instructions of the launch (rocprofv3
+SQ_INSTS_VALU x SQ_WAVES, committed under profiles/) / the launch's duration
+measured here with HIP events; peak = 4 SIMDs x CUs x clock / 4 cycles.
+`hbm_frac_physical` is the launch's measured HBM bytes (FETCH_SIZE and
+WRITE_SIZE, calibrated against a device copy) over its duration and 8 TB/s;
+`effective_frac` the plain per-step byte model (every intermediate written
+and read once in float32) over the duration, which a fused kernel that
+recomputes instead of storing can exceed.
 """
 from __future__ import annotations
 
@@ -15,6 +27,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -24,11 +37,9 @@ if ROOT not in sys.path:
 
 METRIC = "star·λ-bins/s fwd+adjoint, 1M stars→50²×3721 cube; % HBM roofline vs plain"
 HBM_PEAK_GBS = 8000.0
-# Physical HBM bytes of one rbx_cube_bwd launch (star_bwd_reg_kernel +
-# star_grad_kernel) at 1M stars -> 50x50x3721 with the 12x100x5000 grid:
-# rocprofv3 FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), the factors from a 1 GiB
-# device copy (bench.py --calibrate-copy); profiles/hbm_traffic_pmc_1M_50x50x3721.md.
-TRAFFIC_BWD_1M_BYTES = 3.602e9
+CLOCK_GHZ = 2.4          # MI355X peak engine clock
+SIMDS_PER_CU = 4
+CYCLES_PER_VALU = 4      # a wave64 VALU instruction on a 16-lane SIMD
 
 
 def log(*a):
@@ -44,9 +55,17 @@ def algorithmic_bytes(n, n_src, n_tgt, n_cube):
     return fwd, adj
 
 
-def cpu_baseline(w, sample, threads):
+def cpu_baseline(w, sample, threads, reps=5, warmup=1, extinction=None, n_full=None):
     """The oracle's torch-CPU restatement of the plain per-step path (float32,
-    autograd adjoint) on `sample` stars of the same workload, all host cores."""
+    autograd adjoint) on the first `sample` stars of workload w (its full
+    grid, cube, PSF and LSF), all `threads` host cores: `warmup` untimed
+    passes, then the median of `reps` timed passes.  The per-star stages
+    (lookup, mass scaling, Doppler resampling, cube) and the cube stages
+    (PSF, LSF) are timed apart, forward and adjoint each, so the rate can be
+    stated for the workload's own star count n_full: the per-star time
+    scales with the stars, the cube stages do not.
+    Returns a dict of median stage times (s) and the rates at `sample` and
+    at n_full stars (star*lambda-bins/s of forward + adjoint)."""
     import numpy as np
     import torch
 
@@ -55,20 +74,180 @@ def cpu_baseline(w, sample, threads):
     torch.set_num_threads(threads)
     st = {k: v[:sample] for k, v in w.stars.items()}
     f32 = torch.float32
-    ids = torch.tensor(R.square_spaxel_assignment(st["coords"], w.geom["edges"]).astype(np.int64))
-    mass = torch.tensor(st["mass"], dtype=f32, requires_grad=True)
-    age = torch.tensor(st["age"], dtype=f32, requires_grad=True)
-    met = torch.tensor(st["metallicity"], dtype=f32, requires_grad=True)
-    vl = torch.tensor(st["velocity"][:, 2], dtype=f32, requires_grad=True)
-    sp = torch.tensor(w.geom["psf_sigma"], dtype=f32, requires_grad=True)
-    sl = torch.tensor(w.geom["lsf_sigma"], dtype=f32, requires_grad=True)
-    t0 = time.perf_counter()
-    _, _, c2 = RT.full_forward(mass, age, met, vl, ids, sp, sl, w.grid, w.geom, f32)
-    loss = (c2 * c2).sum()
-    loss.backward()
-    dt = time.perf_counter() - t0
-    nt = len(w.geom["wave_target"])
-    return sample * nt / dt, dt
+    geo = w.geom
+    ids = torch.tensor(R.square_spaxel_assignment(st["coords"], geo["edges"]).astype(np.int64))
+    ext = None
+    if extinction is not None:
+        ext = (torch.tensor(extinction[0][:sample], dtype=f32), torch.tensor(extinction[1], dtype=f32))
+    times = {"fwd_star": [], "fwd_cube": [], "adj_cube": [], "adj_star": []}
+    m = geo["psf_size"]
+    for it in range(warmup + reps):
+        mass = torch.tensor(st["mass"], dtype=f32, requires_grad=True)
+        age = torch.tensor(st["age"], dtype=f32, requires_grad=True)
+        met = torch.tensor(st["metallicity"], dtype=f32, requires_grad=True)
+        vl = torch.tensor(st["velocity"][:, 2], dtype=f32, requires_grad=True)
+        sp = torch.tensor(geo["psf_sigma"], dtype=f32, requires_grad=True)
+        sl = torch.tensor(geo["lsf_sigma"], dtype=f32, requires_grad=True)
+        t0 = time.perf_counter()
+        cube = RT.star_cube(mass, age, met, vl, ids, w.grid, geo, f32, extinction=ext)
+        t1 = time.perf_counter()
+        cd = cube.detach().requires_grad_(True)
+        c1 = RT.apply_psf(cd, RT.gaussian_kernel_2d(m, m, sp))
+        c2 = RT.apply_lsf(c1, RT.lsf_kernel(sl, geo["wave_res"]))
+        loss = (c2 * c2).sum()
+        t2 = time.perf_counter()
+        loss.backward()
+        t3 = time.perf_counter()
+        cube.backward(cd.grad)
+        t4 = time.perf_counter()
+        if it >= warmup:
+            for k, v in zip(times, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+                times[k].append(v)
+    med = {k: float(np.median(v)) for k, v in times.items()}
+    nt = len(geo["wave_target"])
+    star_s = med["fwd_star"] + med["adj_star"]
+    cube_s = med["fwd_cube"] + med["adj_cube"]
+    n_full = n_full or sample
+    return {"sample_rate": sample * nt / (star_s + cube_s),
+            "rate": n_full * nt / (star_s * n_full / sample + cube_s),
+            "stage_s": {k: round(v, 4) for k, v in med.items()}}
+
+
+def cpu_leg(w, sample, threads, label, reps=5, warmup=1, n_full=None, **kw):
+    """One cpu_baseline measurement as a JSON-ready dict."""
+    r = cpu_baseline(w, sample, threads, reps, warmup, n_full=n_full, **kw)
+    return {"workload": label, "stars": sample, "stars_rated": n_full or sample,
+            "value": r["rate"], "value_at_sample": r["sample_rate"], "stage_s": r["stage_s"],
+            "reps": reps, "warmup": warmup}
+
+
+def load_counters(name):
+    """The JSON block of profiles/<name>.md: per C-ABI launch, its kernels'
+    waves, VALU instructions per wave and HBM KiB fetched / written per
+    dispatch (rocprofv3 counters, with the commands that made them), and
+    the copy calibration of FETCH_SIZE / WRITE_SIZE."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    return dict(json.loads(m.group(1)), source=f"profiles/{name}.md") if m else None
+
+
+def valu_peak_per_s():
+    import torch
+
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    return SIMDS_PER_CU * cus * CLOCK_GHZ * 1e9 / CYCLES_PER_VALU, cus
+
+
+def roofline_of(counters, call, call_ms, algo_bytes):
+    """Roofline object of one C-ABI launch from its committed counters and
+    its duration measured in this run (None when no counters match)."""
+    if not counters or call not in counters.get("launches", {}):
+        return None
+    L = counters["launches"][call]
+    cal = counters["copy_calibration"]
+    valu = sum(k["waves"] * k["valu_per_wave"] for k in L["kernels"].values())
+    traffic = sum(k["fetch_kib"] * 1024 * cal["fetch_scale"] + k["write_kib"] * 1024 * cal["write_scale"]
+                  for k in L["kernels"].values())
+    peak, cus = valu_peak_per_s()
+    t = call_ms * 1e-3
+    achieved = valu / t
+    return {"bound": "valu", "achieved": round(achieved / 1e9, 2), "peak": round(peak / 1e9, 2),
+            "unit": "G wave-instructions/s", "frac": round(achieved / peak, 4),
+            "traffic": round(traffic), "kernel": call, "kernels": sorted(L["kernels"]),
+            "kernel_ms": round(call_ms, 4), "valu_wave_instructions": round(valu),
+            "hbm_frac_physical": round(traffic / t / (HBM_PEAK_GBS * 1e9), 4),
+            "effective_frac": round(algo_bytes / t / (HBM_PEAK_GBS * 1e9), 4),
+            "algorithmic_bytes_per_launch": algo_bytes, "cus": cus, "clock_ghz": CLOCK_GHZ,
+            "counters": counters["source"]}
+
+
+# Kernels of each C-ABI launch (name prefixes after the namespace).
+LAUNCH_KERNELS = {
+    "fwd": ("band_key_kernel", "batch_keys_kernel", "fill_i32_kernel", "sort_walk_kernel",
+            "scan_reduce_kernel", "scan_top_kernel", "scan_down_kernel", "sort_offsets_kernel",
+            "star_prep_kernel", "chunk_count_kernel", "chunk_fill_kernel", "cube_fused_kernel",
+            "star_norm_kernel", "star_sf_kernel", "cube_accum_kernel", "cube_combine_kernel"),
+    "bwd": ("star_bwd_reg_kernel", "star_bwd_kernel", "star_grad_kernel"),
+}
+
+
+def _kernel_key(name):
+    n = name.split("(")[0].replace("void ", "").strip()
+    return n.split("::")[-1] if "<" not in n else n[n.find("::") + 2:] if "::" in n else n
+
+
+def _pmc_rows(d):
+    import csv
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+        rows += list(csv.DictReader(open(f)))
+    return rows
+
+
+def summarize_counters(args):
+    """Markdown summary (with a JSON block bench.py reads) of rocprofv3 PMC
+    runs of one workload: per C-ABI launch (--launch NAME=COUNT:fwd|bwd, COUNT
+    launches of it in the run) the total SQ_WAVES, SQ_INSTS_VALU, FETCH_SIZE and
+    WRITE_SIZE of its kernels per launch, and the copy calibration."""
+    per = {}
+    for kind, d in (("valu", args.valu_dir), ("fetch", args.fetch_dir), ("write", args.write_dir)):
+        for r in _pmc_rows(d):
+            k = _kernel_key(r["Kernel_Name"])
+            c = r["Counter_Name"]
+            e = per.setdefault(k, {})
+            e[c] = e.get(c, 0.0) + float(r["Counter_Value"])
+            if kind == "valu" and c == "SQ_WAVES":
+                e["dispatches"] = e.get("dispatches", 0) + 1
+
+    def copy_kib(d, counter):
+        vals = [float(r["Counter_Value"]) for r in _pmc_rows(d)
+                if "copyBuffer" in r["Kernel_Name"] and r["Counter_Name"] == counter]
+        return max(vals)
+
+    cal_bytes = 1 << 30
+    fetch_scale = cal_bytes / 1024 / copy_kib(args.copy_fetch_dir, "FETCH_SIZE")
+    write_scale = cal_bytes / 1024 / copy_kib(args.copy_write_dir, "WRITE_SIZE")
+    launches = {}
+    for spec in args.launch:
+        name, rest = spec.split("=")
+        count, kind = rest.split(":")
+        count = int(count)
+        ks = {}
+        for k, e in per.items():
+            base = k.split("<")[0]
+            if base in LAUNCH_KERNELS[kind] and "SQ_WAVES" in e:
+                ks[k] = {"waves": e["SQ_WAVES"] / count,
+                         "valu_per_wave": e["SQ_INSTS_VALU"] / e["SQ_WAVES"],
+                         "fetch_kib": e.get("FETCH_SIZE", 0.0) / count,
+                         "write_kib": e.get("WRITE_SIZE", 0.0) / count,
+                         "dispatches_per_launch": e["dispatches"] / count}
+        launches[name] = {"kernels": ks}
+    label = args.workload_label or args.workload
+    data = {"workload": label, "launches": launches,
+            "copy_calibration": {"bytes": cal_bytes, "fetch_scale": round(fetch_scale, 5),
+                                 "write_scale": round(write_scale, 5)}}
+    out = [f"# Counters per launch: {label}", "",
+           "Commands (from /tmp with TMPDIR=/tmp, after `python -m rubix_amd.build`):", "", "```"]
+    out += [c for c in args.command]
+    out += ["```", "",
+            f"Copy calibration: a 1 GiB device copy reads FETCH_SIZE x {fetch_scale:.4f} and "
+            f"WRITE_SIZE x {write_scale:.4f} = 1 GiB per dispatch; the bytes below are scaled by "
+            "these factors.", "",
+            "| launch | kernel | waves | VALU / wave | VALU wave-instr | HBM read B | HBM written B |",
+            "|---|---|---|---|---|---|---|"]
+    for ln, L in launches.items():
+        for k, e in sorted(L["kernels"].items(), key=lambda kv: -kv[1]["waves"] * kv[1]["valu_per_wave"]):
+            out.append(f"| {ln} | {k} | {e['waves']:.4g} | {e['valu_per_wave']:.1f} | "
+                       f"{e['waves'] * e['valu_per_wave']:.4g} | {e['fetch_kib'] * 1024 * fetch_scale:.4g} | "
+                       f"{e['write_kib'] * 1024 * write_scale:.4g} |")
+    out += ["", "```json", json.dumps(data, indent=1), "```"]
+    print("\n".join(out))
 
 
 def _model_name():
@@ -126,7 +305,11 @@ def main():
                          "apply_noise step on a 50x50x3721 cube; dust: the single workload "
                          "with 1M gas cells and dust extinction (fused calc_dusty_ifu)")
     ap.add_argument("--n-stars", type=int, default=None)
-    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--workload-label", dest="workload_label", default=None)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="stars of the workload timed on the host (default per workload)")
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="timed host passes after one warm-up (median reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="timed steps of the plain per-step GPU path (0 = skip)")
@@ -135,10 +318,29 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (twice): the reference "
                          "workload for scaling the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--counters-run", action="store_true",
+                    help="one forward (the target) and one forward + adjoint step of the "
+                         "workload and nothing else: the run rocprofv3 PMC passes profile")
+    ap.add_argument("--summarize-counters", action="store_true",
+                    help="print the counters summary of PMC output directories (no GPU)")
+    ap.add_argument("--valu-dir")
+    ap.add_argument("--fetch-dir")
+    ap.add_argument("--write-dir")
+    ap.add_argument("--copy-fetch-dir")
+    ap.add_argument("--copy-write-dir")
+    ap.add_argument("--launch", action="append", default=[],
+                    help="NAME=COUNT:fwd|bwd for --summarize-counters")
+    ap.add_argument("--command", action="append", default=[],
+                    help="profiling command line recorded in the summary")
     args = ap.parse_args()
+    if args.summarize_counters:
+        summarize_counters(args)
+        return
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.counters_run:
+        args.steps, args.warmup, args.no_cpu, args.plain_steps = 1, 0, True, 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is a "
               "design (sharding by stars + a lambda-slab reduce-scatter) and is not built.",
@@ -246,19 +448,21 @@ def run_single(args, dust=False):
 
     fwd_b, adj_b = algorithmic_bytes(n, n_src, nt, nb * nb * nt)
     step_frac = (fwd_b + adj_b) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
-    # dominant launch: the per-star adjoint rbx_cube_bwd (star_bwd_kernel, then
-    # the small per-star star_grad_kernel)
+    # dominant launch: the per-star adjoint rbx_cube_bwd (star_bwd_reg_kernel,
+    # then the small per-star star_grad_kernel); the forward launch second
     bwd_name = "rbx_cube_bwd_ext" if dust else "rbx_cube_bwd"
-    bwd_ms = per_call.get(bwd_name, float("nan"))
-    bwd_bytes = n * (24 * n_src + 8 * nt)
-    achieved = bwd_bytes / (bwd_ms * 1e-3) / 1e9
-    measured = (n, nb, nt, n_src) == (1_000_000, 50, 3721, 5000) and not dust
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": TRAFFIC_BWD_1M_BYTES if measured else None,
-                "kernel": f"{bwd_name} (star_bwd_reg_kernel + star_grad_kernel)",
-                "kernel_ms": round(bwd_ms, 4), "algorithmic_bytes_per_launch": bwd_bytes,
-                "step_effective_frac": round(step_frac, 4)}
+    fwd_name = "rbx_cube_fwd_ext" if dust else "rbx_cube_fwd"
+    counters = None
+    if (n, nb, nt, n_src) == (1_000_000, 50, 3721, 5000):
+        counters = load_counters("counters_dust_1M_gas1M_50x50x3721" if dust
+                                 else "counters_1M_50x50x3721")
+    roofline = roofline_of(counters, bwd_name, per_call.get(bwd_name, float("nan")),
+                           n * (24 * n_src + 8 * nt))
+    if roofline is not None:
+        roofline["forward_launch"] = roofline_of(counters, fwd_name,
+                                                 per_call.get(fwd_name, float("nan")),
+                                                 n * (16 * n_src + 8 * nt))
+        roofline["step_effective_frac"] = round(step_frac, 4)
     split = {"fwd_ms_median": round(float(np.median(fwd_ms)), 4),
              "adj_ms_median": round(float(np.median(adj_ms)), 4),
              "step_ms_median": round(float(np.median(np.add(fwd_ms, adj_ms))), 4)}
@@ -270,22 +474,42 @@ def run_single(args, dust=False):
         torch.cuda.reset_peak_memory_stats(dev)
         p_el, p_fwd, p_adj = timed_leg(make_step(plain), 1, args.plain_steps)
         p_ms = p_el / args.plain_steps * 1e3
+        plain_roof_ms = (fwd_b + adj_b) / (HBM_PEAK_GBS * 1e9) * 1e3
         plain_res = {"ms_per_step": round(p_ms, 3), "value": n * nt / (p_el / args.plain_steps),
                      "fwd_ms_median": round(float(np.median(p_fwd)), 3),
                      "adj_ms_median": round(float(np.median(p_adj)), 3),
                      "hbm_frac": round((fwd_b + adj_b) / (p_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                      "peak_hbm_gb": round(torch.cuda.max_memory_allocated(dev) / 1e9, 2),
-                     "fused_speedup": round(p_ms / ms, 3), "steps": args.plain_steps}
+                     "fused_speedup": round(p_ms / ms, 3),
+                     "plain_roofline_ms": round(plain_roof_ms, 3),
+                     "speedup_vs_plain_roofline": round(plain_roof_ms / ms, 3),
+                     "steps": args.plain_steps}
         log("plain:", json.dumps(plain_res))
 
     cpu = None
-    if not args.no_cpu and not dust:
+    if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
-        rate, dt = cpu_baseline(w, args.cpu_sample, threads)
-        cpu = {"value": rate, "unit": "star·λ-bins/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_sample} stars of the same workload (full SSP grid, 50x50x3721 cube, "
-                         f"PSF + LSF), float32 torch-CPU restatement of the plain per-step path, "
-                         f"forward + autograd adjoint, one pass {dt:.1f}s on {_model_name()}"}
+        ext_host = None
+        if dust:
+            av, axav = ext_of()
+            ext_host = (av.cpu().numpy(), axav.cpu().numpy())
+        sample = min(args.cpu_sample or 25_000, n)
+        legs = [cpu_leg(w, sample, threads,
+                        f"{sample} stars of this workload (full grid, cube, PSF, LSF)",
+                        reps=args.cpu_reps, extinction=ext_host, n_full=n)]
+        if not dust:
+            w1 = S.config1()
+            legs.append(cpu_leg(w1, w1.n_stars, threads,
+                                "the whole small workload: 10k stars -> 25x25x1000, SSP 6x20x1000",
+                                reps=args.cpu_reps))
+        cpu = {"value": legs[0]["value"], "unit": "star·λ-bins/s", "cores": threads, "kind": "port",
+               "sample": (f"{sample} stars of the same workload: float32 torch-CPU restatement "
+                          f"of the plain per-step path"
+                          f"{' with dust extinction (A_V per star given)' if dust else ''}, "
+                          f"forward + autograd adjoint, 1 warm-up + median of {args.cpu_reps} "
+                          f"passes on {threads} threads of {_model_name()}; per-star stages "
+                          f"scaled to the {n} stars, the cube stages (PSF, LSF) once"),
+               "legs": legs, "gpu_over_cpu": round(value / legs[0]["value"], 1)}
     workload = (f"single galaxy, {n} stars -> {nb}x{nb}x{nt} cube, SSP 12x100x{n_src}, "
                 "PSF 15x15 sigma 4 px, LSF sigma 1.0 A; fused forward + adjoint of chi^2")
     if dust:
@@ -320,7 +544,7 @@ def run_large(args):
     import numpy as np
     import torch
 
-    from rubix_amd import ops, synthetic as S
+    from rubix_amd import _lib, ops, synthetic as S
     from rubix_amd.autograd import render, render_plain
 
     dev = torch.device("cuda", 0)
@@ -372,10 +596,23 @@ def run_large(args):
     torch.cuda.synchronize()
     torch.cuda.reset_peak_memory_stats(dev)
     _, step = make_step(t, sp, sl, render, tgt, wt)
+    _lib.TIMER.start()
     elapsed, fwd_ms, adj_ms = timed_leg(step, args.warmup, args.steps)
+    calls = _lib.TIMER.stop()
+    if args.counters_run:
+        return
+    per_call = {k: float(np.median(v[-args.steps:])) for k, v in calls.items()}
     ms = elapsed / args.steps * 1e3
     peak = torch.cuda.max_memory_allocated(dev) / 1e9
     fwd_b, adj_b = algorithmic_bytes(n, n_src, nt, nb * nb * nt)
+    counters = load_counters("counters_10M_150x150x3721") if (n, nb, nt) == (10_000_000, 150, 3721) \
+        else None
+    roofline = roofline_of(counters, "rbx_cube_bwd", per_call.get("rbx_cube_bwd", float("nan")),
+                           n * (24 * n_src + 8 * nt))
+    if roofline is not None:
+        roofline["forward_launch"] = roofline_of(counters, "rbx_cube_fwd",
+                                                 per_call.get("rbx_cube_fwd", float("nan")),
+                                                 n * (16 * n_src + 8 * nt))
     del t, sp, sl, tgt, wt
     torch.cuda.empty_cache()
 
@@ -395,11 +632,26 @@ def run_large(args):
         del t, sp, sl, tgt, wt
         torch.cuda.empty_cache()
     ratio = overhead["recompute"]["adj_ms_median"] / overhead["stored"]["adj_ms_median"]
+    value = n * nt / (elapsed / args.steps)
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        sample = min(args.cpu_sample or 10_000, n)
+        leg = cpu_leg(w, sample, threads, f"{sample} stars of this workload (full grid, cube, PSF, "
+                      "LSF)", reps=2, n_full=n)
+        cpu = {"value": leg["value"], "unit": "star·λ-bins/s", "cores": threads, "kind": "port",
+               "sample": (f"{sample} stars of the same workload: float32 torch-CPU restatement of "
+                          f"the plain per-step path, forward + autograd adjoint, 1 warm-up + median "
+                          f"of 2 passes on {threads} threads of {_model_name()}; per-star stages "
+                          f"scaled to the {n} stars, the cube stages (PSF, LSF) once"),
+               "legs": [leg], "gpu_over_cpu": round(value / leg["value"], 1)}
     res = {
         "metric": "star·λ-bins/s fwd+adjoint, 10M stars→150²×3721 cube (recompute)",
-        "value": n * nt / (elapsed / args.steps), "unit": "star·λ-bins/s", "n_gpus": 1,
+        "value": value, "unit": "star·λ-bins/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "roofline": roofline, "cpu_baseline": cpu,
+        "per_call_ms": {k: round(v, 4) for k, v in per_call.items()},
         "config": {"workload": f"single galaxy, {n} stars -> {nb}x{nb}x{nt} cube, SSP 6x20x{n_src}, "
                                "PSF 11x11 sigma 3 px, LSF sigma 1.2 A; fused forward + adjoint of chi^2",
                    "n_stars": n, "n_spaxel": nb * nb, "n_target": nt, "n_source": n_src},
@@ -424,7 +676,7 @@ def run_batched(args):
     import numpy as np
     import torch
 
-    from rubix_amd import autotune, ops, synthetic as S
+    from rubix_amd import _lib, autotune, ops, synthetic as S
     from rubix_amd.autograd import render, render_plain
     from rubix_amd.batch import GalaxyBatch, GraphedStep
 
@@ -462,6 +714,10 @@ def run_batched(args):
         cot = 2.0 * (cube.detach() - tgt) * wt
         torch.autograd.grad(cube, inputs, grad_outputs=cot)
 
+    if args.counters_run:
+        eager_step(ops.Tuning())
+        torch.cuda.synchronize()
+        return
     key = autotune.problem_key(L.n_total, L.n_rows, nt, n_src, dev)
     t0 = time.perf_counter()
     if args.no_tune:
@@ -470,7 +726,10 @@ def run_batched(args):
         best, seen = autotune.autotune(eager_step, key, reps=1, log=log)
     log(f"autotune {time.perf_counter() - t0:.1f}s -> {best}")
     state["tune"] = best
+    _lib.TIMER.start()
     eager_ms = autotune.time_ms(eager_step, best, reps=3)
+    calls = _lib.TIMER.stop()
+    per_call = {k: float(np.median(v[-3:])) for k, v in calls.items()}
     default_ms = autotune.time_ms(eager_step, ops.Tuning(), reps=3)
 
     step = GraphedStep(rend, inputs, tgt, wt)
@@ -506,11 +765,47 @@ def run_batched(args):
     loop(render)  # warm-up of allocations
     fused_loop_ms = loop(render)
     plain_loop_ms = loop(render_plain) if args.plain_steps > 0 else None
+    value = work / (elapsed / args.steps)
+    counters = load_counters("counters_batched_64gal_3721") if not args.n_stars else None
+    roofline = roofline_of(counters, "rbx_batch_bwd", per_call.get("rbx_batch_bwd", float("nan")),
+                           L.n_total * (24 * n_src + 8 * nt))
+    if roofline is not None:
+        roofline["forward_launch"] = roofline_of(counters, "rbx_batch_fwd",
+                                                 per_call.get("rbx_batch_fwd", float("nan")),
+                                                 L.n_total * (16 * n_src + 8 * nt))
+        roofline["timing"] = "launch durations from the eager step (a graph replay is one launch)"
+    cpu = None
+    if not args.no_cpu:
+        # one galaxy of each grid size, its per-star stages scaled to the
+        # stars of all galaxies of that size and its cube stages once per galaxy
+        threads = len(os.sched_getaffinity(0))
+        legs, total_s = [], 0.0
+        for nb in sorted({w.geom["nb"] for w in ws}):
+            members = [w for w in ws if w.geom["nb"] == nb]
+            rep = max(members, key=lambda w: w.n_stars)
+            sample = min(args.cpu_sample or 5_000, rep.n_stars)
+            r = cpu_baseline(rep, sample, threads, reps=2)
+            st = r["stage_s"]
+            n_class = sum(w.n_stars for w in members)
+            t_class = ((st["fwd_star"] + st["adj_star"]) * n_class / sample
+                       + (st["fwd_cube"] + st["adj_cube"]) * len(members))
+            total_s += t_class
+            legs.append({"nb": nb, "galaxies": len(members), "stars": n_class,
+                         "sample_stars": sample, "stage_s": st, "class_s": round(t_class, 3)})
+        cpu = {"value": work / total_s, "unit": "star·λ-bins/s", "cores": threads, "kind": "port",
+               "sample": (f"per grid size one galaxy's first {args.cpu_sample or 5000} stars: float32 "
+                          f"torch-CPU restatement of the plain per-step path, forward + autograd "
+                          f"adjoint, 1 warm-up + median of 2 passes on {threads} threads of "
+                          f"{_model_name()}; per-star stages scaled to the class's stars, cube "
+                          f"stages once per galaxy, summed over the batch"),
+               "legs": legs, "gpu_over_cpu": round(value / (work / total_s), 1)}
     res = {
         "metric": "star·λ-bins/s fwd+adjoint, 64 ragged galaxies in one pass (HIP graph)",
-        "value": work / (elapsed / args.steps), "unit": "star·λ-bins/s", "n_gpus": 1,
+        "value": value, "unit": "star·λ-bins/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "roofline": roofline, "cpu_baseline": cpu,
+        "per_call_ms_eager": {k: round(v, 4) for k, v in per_call.items()},
         "config": {"workload": f"{L.n_gal} galaxies, {L.n_total} stars, {L.n_rows} cube rows x {nt}, "
                                f"SSP 6x20x{n_src}, PSF 11x11, LSF sigma 1.2 A; chi^2 forward + adjoint",
                    "n_galaxies": L.n_gal, "n_stars": L.n_total, "n_rows": L.n_rows,

@@ -445,6 +445,11 @@ def apply_lsf(datacube, lsf_sigma, wave_resolution, extend_factor=12, dtype=None
 
 
 def apply_lsf_kernel(datacube, k, extend_factor=12):
+    """Full convolution of every spectrum with k, cropped to
+    [extend_factor, n + L - 1 - extend_factor) (rubix/telescope/lsf/lsf.py:61-65).
+    With an even tap count L that crop holds n + 1 bins and the reference's
+    reshape to the cube raises (lsf.py:102-105); the first n bins are returned
+    here (the kernel's x = 0 tap on the input bin, as for odd L)."""
     cube = np.asarray(datacube)
     shape = cube.shape
     spec = cube.reshape(-1, shape[-1])
@@ -454,7 +459,8 @@ def apply_lsf_kernel(datacube, k, extend_factor=12):
     for t in range(L):
         full[:, t:t + n] += k[t] * spec
     end = n + L - 1 - extend_factor
-    return full[:, extend_factor:end].reshape(shape)
+    crop = full[:, extend_factor:end]
+    return crop[:, :n].reshape(shape)
 
 
 # ---------------------------------------------------------------- noise

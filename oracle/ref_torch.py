@@ -161,14 +161,18 @@ def lsf_kernel(sigma, wave_res, factor=12):
 
 
 def apply_lsf(cube, k, factor=12):
-    """rubix/telescope/lsf/lsf.py:29-105 (full convolution + crop)."""
+    """rubix/telescope/lsf/lsf.py:29-105 (full convolution + crop
+    [factor, n + L - 1 - factor)).  An even tap count L (the float arange of
+    the kernel grid gives 26 taps for wave_res 1.6 or 2.7) makes that crop
+    n + 1 long, which the reference cannot reshape to the cube (it raises);
+    here the first n bins of the crop are returned, the LSF centred on the
+    x = 0 tap like the odd case."""
     shape = cube.shape
     spec = cube.reshape(-1, shape[-1])
     n = spec.shape[1]
     L = k.numel()
     full = F.conv1d(spec[:, None, :], torch.flip(k, dims=(0,))[None, None, :], padding=L - 1)
-    end = n + L - 1 - factor
-    return full[:, 0, factor:end].reshape(shape)
+    return full[:, 0, factor:factor + n].reshape(shape)
 
 
 def reference_knots(ssp_wave32, vel_los, a):
@@ -184,12 +188,16 @@ def reference_knots(ssp_wave32, vel_los, a):
     return w + (w32 - w).detach()
 
 
-def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots=True):
+def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots=True, extinction=None):
     """Raw datacube (before PSF/LSF) from per-star tensors, the
     calculate_spectra -> scale_spectrum_by_mass ->
     doppler_shift_and_resampling -> calculate_datacube chain
     (rubix/core/ifu.py:63-121,149-157,269-293,327-339).  ``ref_knots``: see
-    reference_knots (float64 evaluation of the reference's float32 knots)."""
+    reference_knots (float64 evaluation of the reference's float32 knots).
+    ``extinction``: optional (av per star, A(lambda)/A(V) per target bin)
+    tensors; the resampled spectra are multiplied by 10^(-0.4 av axav) before
+    the cube (calc_dusty_ifu's calculate_extinction step,
+    rubix/spectra/dust/dust_extinction.py:330-358)."""
     axm = torch.as_tensor(grid["ax_met"], dtype=dtype)
     axa = torch.as_tensor(grid["ax_age"], dtype=dtype)
     flux = torch.as_tensor(grid["flux"], dtype=dtype)
@@ -211,16 +219,21 @@ def star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots=True):
         w = ssp_wave[None, :] * a[:, None]
     tw = torch.as_tensor(geom["wave_target"], dtype=dtype)
     res = resample(spec, w, tw)
+    if extinction is not None:
+        av, axav = extinction
+        res = res * torch.pow(torch.tensor(10.0, dtype=res.dtype),
+                              -0.4 * av.to(res.dtype)[:, None] * axav.to(res.dtype)[None, :])
     nb = len(geom["edges"]) - 1
     return calculate_cube(res, ids, nb)
 
 
 def full_forward(mass, age, met, vel_los, ids, sigma_psf, sigma_lsf, grid, geom,
-                 dtype=torch.float64, ref_knots=True, psf="direct"):
+                 dtype=torch.float64, ref_knots=True, psf="direct", extinction=None):
     """Whole path with tensors for every differentiable input; returns
     (raw cube, cube after PSF, cube after LSF).  psf="separable": the PSF as
-    two 1D passes (equal in exact arithmetic), for large cubes."""
-    cube = star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots)
+    two 1D passes (equal in exact arithmetic), for large cubes; extinction:
+    see star_cube."""
+    cube = star_cube(mass, age, met, vel_los, ids, grid, geom, dtype, ref_knots, extinction)
     m = geom["psf_size"]
     if psf == "separable":
         c1 = apply_psf_separable(cube, gaussian_taps_1d(m, sigma_psf))

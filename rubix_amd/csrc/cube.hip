@@ -104,11 +104,9 @@ __global__ void star_prep_kernel(SspView g, rbx_target tg, rbx_stars st,
     const bool keep = c.inside && key >= 0;
     s.row = keep ? c.row : -1;
     for (int q = 0; q < 4; ++q) s.w[q] = c.w[q];
-    // d w_c / d(log Z) and d w_c / d(log t) for the adjoint's combined rows
-    for (int q = 0; q < 4; ++q) {
-      gradw[r * 8 + q] = c.gu[q];
-      gradw[r * 8 + 4 + q] = c.gs[q];
-    }
+    // the fractional cell position the adjoint's combined rows use
+    gradw[r * 2] = c.fu;
+    gradw[r * 2 + 1] = c.fs;
     // in-range window of the mask (w >= min target) & (w <= max target)
     const int klo_le = locate(g, tg.wmin, a, s.inv_a);
     int klo = (klo_le >= 0 && knot(g, klo_le, a) == tg.wmin) ? klo_le : klo_le + 1;
@@ -856,6 +854,8 @@ __global__ void cube_combine_kernel(const double* __restrict__ partial,
 // Each lane accumulates its bins in float32; the wave's 64 lane sums are
 // added in double in a fixed order (lds_sum), so the Doppler sums, which
 // cancel strongly across bins, lose nothing beyond the float32 per-bin terms.
+constexpr int kVSums = 5;                // value sums per (star, slab): Vg_w, Vg_u, Vg_s, Vd_u, Vd_s
+constexpr int kASums = 2;                // Doppler sums per (star, slab): Ag_w, Ad_w
 constexpr int kRedRow = 68;              // LDS floats per value row (64 lanes, 16-byte rows)
 constexpr int kRedFloats = 8 * kRedRow;  // one wavefront's reduction buffer
 
@@ -889,12 +889,58 @@ __device__ inline double lds_sum(const float v[K], float* buf, int lane) {
   return acc;
 }
 
+// The three combined rows of one knot from its four corner values, in the
+// difference form of Cell (A = f10 - f00, C = f01 - f00, E = f11 - f10 -
+// f01 + f00): w = sum_c w_c f_c, and u, s the corner-derivative sums
+// without their reciprocal cell sizes (star_grad_kernel applies those).
+struct RowVals {
+  float w, u, s;
+};
+
+__device__ inline RowVals combine_rows(float f0, float f1, float f2, float f3, float fu, float fs) {
+  const float A = f1 - f0, C = f2 - f0, E = (f3 - f1) - C;
+  RowVals r;
+  r.s = fmaf(fu, E, C);
+  r.u = fmaf(fs, E, A);
+  r.w = fmaf(fs, r.s, fmaf(fu, A, f0));
+  return r;
+}
+
+// One bin of star_bwd_kernel's sums, from gathered template pairs.
+struct BwdRows {
+  float fu, fs;  // fractional cell position (Cell::fu, Cell::fs)
+};
+
+__device__ inline void bwd_bin_gather(const SspView& g, const Rows& R, const StarState& s,
+                                      const BwdRows& B, const Shift& h, float x, float gj,
+                                      float dj, float v[7]) {
+  const Knots K = knots_of<true>(g, x, h);
+  const float gk = -gj * K.kappa, dk = -dj * K.kappa;
+  Pair f[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) f[c] = ld_pair(R.f[c], K.i0);
+  const RowVals a = combine_rows(f[0].lo, f[1].lo, f[2].lo, f[3].lo, B.fu, B.fs);
+  const RowVals b = combine_rows(f[0].hi, f[1].hi, f[2].hi, f[3].hi, B.fu, B.fs);
+  const float dw = b.w - a.w;
+  const float Rw = a.w + K.theta * dw;
+  const float Ru = a.u + K.theta * (b.u - a.u);
+  const float Rs = a.s + K.theta * (b.s - a.s);
+  v[0] += gj * Rw;
+  v[1] += gj * Ru;
+  v[2] += gj * Rs;
+  v[3] += dj * Ru;
+  v[4] += dj * Rs;
+  v[5] += gk * dw;
+  v[6] += dk * dw;
+}
+
 // Slab-mapped like star_norm_kernel.  The per-star gradients need the four
 // corner rows only through three weightings: the bilinear weights w_c and
-// their derivatives gu_c = dw_c/d(log Z), gs_c = dw_c/d(log t).  Per bin the
-// kernel gathers the four corner pairs (8-byte loads from scalar bases),
-// forms the combined rows R_w, R_u, R_s at the bin and accumulates the seven
-// sums star_grad_kernel uses:
+// their derivatives gu_c = dw_c/d(log Z), gs_c = dw_c/d(log t), formed as
+// combine_rows does from corner differences.  Per bin the kernel gathers the
+// four corner pairs (8-byte loads from scalar bases), forms the combined rows
+// R_w, R_u, R_s at the bin and accumulates the seven sums star_grad_kernel
+// uses:
 //   Vg_w, Vg_u, Vg_s = sum_j g_j R_{w,u,s}(j),  Vd_u, Vd_s = sum_j d_j R_{u,s}(j),
 //   Ag_w, Ad_w = sum_j {g_j, d_j} dR_w(j)/da      (d = dtarget),
 // float32 per lane, added across lanes in double in a fixed order; vpart
@@ -921,12 +967,9 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
       const Rows R = rows_of(g, __builtin_amdgcn_readfirstlane(s.row));
       const float* grow = dcube + int64_t(s.spaxel) * tg.n;
       const Shift h = shift_of(g, s.a, s.inv_a);
-      float gu[4], gs[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        gu[c] = gradw[t.r * 8 + c];
-        gs[c] = gradw[t.r * 8 + 4 + c];
-      }
+      BwdRows B;
+      B.fu = gradw[t.r * 2];
+      B.fs = gradw[t.r * 2 + 1];
       const uint32_t j0 = uint32_t(t.slab * slab_bins);
       const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
 #pragma unroll 2
@@ -935,75 +978,17 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
         float gj = ld_at(grow, j);
         // with dust the cube holds sf P_j e_j: every g-sum takes g_j e_j
         if (kDust && s.av != 0.f) gj *= dust_factor(dust_tau(axav, j), s.av);
-        const float dj = xd.hi;
-        const Knots K = knots_of<true>(g, xd.lo, h);
-        const float gk = -gj * K.kappa, dk = -dj * K.kappa;
-        float w0 = 0.f, w1 = 0.f, u0 = 0.f, u1 = 0.f, s0 = 0.f, s1 = 0.f;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const Pair f = ld_pair(R.f[c], K.i0);
-          w0 += s.w[c] * f.lo;
-          w1 += s.w[c] * f.hi;
-          u0 += gu[c] * f.lo;
-          u1 += gu[c] * f.hi;
-          s0 += gs[c] * f.lo;
-          s1 += gs[c] * f.hi;
-        }
-        const float dw = w1 - w0;
-        const float Rw = w0 + K.theta * dw;
-        const float Ru = u0 + K.theta * (u1 - u0);
-        const float Rs = s0 + K.theta * (s1 - s0);
-        v[0] += gj * Rw;
-        v[1] += gj * Ru;
-        v[2] += gj * Rs;
-        v[3] += dj * Ru;
-        v[4] += dj * Rs;
-        v[5] += gk * dw;
-        v[6] += dk * dw;
+        bwd_bin_gather(g, R, s, B, h, xd.lo, gj, xd.hi, v);
       }
     }
     const double tot = lds_sum<7>(v, red[t.wave], t.lane);
     const int q = t.lane >> 3;
     if (live && (t.lane & 7) == 0 && q < 7) {
-      const int64_t base = (t.r * kSlabs + t.slab) * 8;
-      if (q < 5) vpart[base + q] = float(tot);
-      else apart[base + q - 5] = tot;
+      const int64_t it = t.r * kSlabs + t.slab;
+      if (q < kVSums) vpart[it * kVSums + q] = float(tot);
+      else apart[it * kASums + q - kVSums] = tot;
     }
   }
-}
-
-// One bin of star_bwd_kernel's sums, from gathered template pairs.
-struct BwdRows {
-  float gu[4], gs[4];
-};
-
-__device__ inline void bwd_bin_gather(const SspView& g, const Rows& R, const StarState& s,
-                                      const BwdRows& B, const Shift& h, float x, float gj,
-                                      float dj, float v[7]) {
-  const Knots K = knots_of<true>(g, x, h);
-  const float gk = -gj * K.kappa, dk = -dj * K.kappa;
-  float w0 = 0.f, w1 = 0.f, u0 = 0.f, u1 = 0.f, s0 = 0.f, s1 = 0.f;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const Pair f = ld_pair(R.f[c], K.i0);
-    w0 += s.w[c] * f.lo;
-    w1 += s.w[c] * f.hi;
-    u0 += B.gu[c] * f.lo;
-    u1 += B.gu[c] * f.hi;
-    s0 += B.gs[c] * f.lo;
-    s1 += B.gs[c] * f.hi;
-  }
-  const float dw = w1 - w0;
-  const float Rw = w0 + K.theta * dw;
-  const float Ru = u0 + K.theta * (u1 - u0);
-  const float Rs = s0 + K.theta * (s1 - s0);
-  v[0] += gj * Rw;
-  v[1] += gj * Ru;
-  v[2] += gj * Rs;
-  v[3] += dj * Ru;
-  v[4] += dj * Rs;
-  v[5] += gk * dw;
-  v[6] += dk * dw;
 }
 
 // v of lane l (0 <= l < 64) by one ds_bpermute (no lane wrapping needed).
@@ -1021,6 +1006,16 @@ __device__ inline float lane_of(float v, int l) {
 // win_knots, searched on the permuted knots with a wave-uniform correction
 // loop.  A group whose knots do not fit one lane each, or where a bin's
 // segment would leave the window, is done with the gathers instead.
+//
+// Full groups of a slab strictly inside the grid run first on a short path:
+// the locator guesses are monotone in the bin, so every lane's guess lies in
+// the window [1, span - 3] without clamps, a lane that does not end on
+// w_l <= x < w_r after the correction steps is a segment outside the window,
+// and no grid-end case can apply.  A group that does not fit that path
+// (window over 64 knots or reaching a grid end, a segment outside it) is
+// marked and redone after the loop with the general path; nothing of it was
+// added, so each group's terms are added exactly once, in group order within
+// each of the two loops.
 template <int W, bool kDust>
 __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
     SspView g, rbx_target tg, int64_t n, int64_t n_items, const StarState* __restrict__ state,
@@ -1042,11 +1037,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
       const float* grow = dcube + int64_t(s.spaxel) * tg.n;
       const Shift h = shift_of(g, s.a, s.inv_a);
       BwdRows B;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        B.gu[c] = gradw[t.r * 8 + c];
-        B.gs[c] = gradw[t.r * 8 + 4 + c];
-      }
+      B.fu = gradw[t.r * 2];
+      B.fs = gradw[t.r * 2 + 1];
       const uint32_t j0 = uint32_t(t.slab * slab_bins);
       const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
       // interior slab: every bin's segment has knots in [1, n - 2] (x ascending
@@ -1058,11 +1050,68 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         const float wn = mul_nocontract(ld_at(g.wave, uint32_t(n_w - 2)), h.a);
         interior = n_w >= 4 && ld_at(tg.wave, j0) >= w1 && ld_at(tg.wave, j1 - 1) < wn;
       }
-      auto groups = [&](auto interior_c) {
+      // the short path of a full interior group; false (nothing added) when
+      // the group needs the general path
+      auto fast_group = [&](uint32_t jb) -> bool {
+        const uint32_t j = jb + uint32_t(t.lane);
+        const Pair xd = ld_pair(tg.pairs, 2 * j);
+        const float x = xd.lo, dj = xd.hi;
+        float gj = ld_at(grow, j);
+        if (kDust && s.av != 0.f) gj *= dust_factor(dust_tau(axav, j), s.av);
+        const float fm = __builtin_amdgcn_fmed3f(fmaf(x, h.c1, h.c0), 0.f, float(g.n_loc - 1));
+        const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
+        const int wb = __builtin_amdgcn_readfirstlane(k0) - 1;
+        const int we = __builtin_amdgcn_readlane(k0, 63) + 2;
+        const int span = we - wb + 1;
+        if (wb < 0 || we > n_w - 1 || span > 64) return false;
+        const int kl = wb + t.lane;
+        const uint32_t kk = uint32_t(kl < we ? kl : we);
+        const float lamk = ld_at(g.wave, kk);
+        const RowVals rv = combine_rows(ld_at(R.f[0], kk), ld_at(R.f[1], kk), ld_at(R.f[2], kk),
+                                        ld_at(R.f[3], kk), B.fu, B.fs);
+        const float y = x * h.inv_a;
+        int li = k0 - wb;  // in [1, span - 3]: the guesses are monotone in the bin
+        li += lane_of(lamk, li + 1) <= y ? 1 : 0;
+        float l0 = lane_of(lamk, li), l1 = lane_of(lamk, li + 1);
+        float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
+        if (__ballot(x >= wr || x < wl)) {  // rare: correction steps inside the window
+          bool step = (x >= wr && li < span - 2) || (x < wl && li > 0);
+          while (__ballot(step)) {
+            const int nl = step ? li + (x >= wr ? 1 : -1) : li;
+            const float a0 = lane_of(lamk, nl), a1 = lane_of(lamk, nl + 1);
+            if (step) {
+              li = nl;
+              l0 = a0;
+              l1 = a1;
+              wl = mul_nocontract(l0, h.a);
+              wr = mul_nocontract(l1, h.a);
+            }
+            step = (x >= wr && li < span - 2) || (x < wl && li > 0);
+          }
+          // still outside: the segment lies past the window's edge
+          if (__ballot(x >= wr || x < wl)) return false;
+        }
+        const float r = __builtin_amdgcn_rcpf(wr - wl);
+        const float th = (x - wl) * r;
+        const float kap = (l0 + th * (l1 - l0)) * r;
+        const float w0 = lane_of(rv.w, li), w1 = lane_of(rv.w, li + 1);
+        const float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
+        const float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
+        const float dw = w1 - w0;
+        const float Ruv = u0 + th * (u1 - u0);
+        const float Rsv = s0 + th * (s1 - s0);
+        v[0] += gj * (w0 + th * dw);
+        v[1] += gj * Ruv;
+        v[2] += gj * Rsv;
+        v[3] += dj * Ruv;
+        v[4] += dj * Rsv;
+        v[5] += (-gj * kap) * dw;
+        v[6] += (-dj * kap) * dw;
+        return true;
+      };
+      // the general path of one group (any slab, full or the last partial one)
+      auto group = [&](uint32_t jb, auto interior_c, auto full_c) {
         constexpr bool kIn = decltype(interior_c)::value;
-      // a full group (every lane's bin below j1) runs without the per-lane
-      // validity selects and the masked cotangent loads
-      auto group = [&](uint32_t jb, auto full_c) {
         constexpr bool kFull = decltype(full_c)::value;
         const uint32_t j = jb + uint32_t(t.lane);
         const bool valid = kFull || j < j1;
@@ -1087,14 +1136,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
           const int kl = wb + t.lane;
           const uint32_t kk = uint32_t(kl < we ? kl : we);
           const float lamk = ld_at(g.wave, kk);
-          float rw = 0.f, ru = 0.f, rs = 0.f;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const float f = ld_at(R.f[c], kk);
-            rw += s.w[c] * f;
-            ru += B.gu[c] * f;
-            rs += B.gs[c] * f;
-          }
+          const RowVals rv = combine_rows(ld_at(R.f[0], kk), ld_at(R.f[1], kk),
+                                          ld_at(R.f[2], kk), ld_at(R.f[3], kk), B.fu, B.fs);
           // exact segment on the window's knots (as win_knots)
           const float y = x * h.inv_a;
           int li = k0 - wb;
@@ -1143,9 +1186,9 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
             const bool right = !kIn && (i == n_w - 1) && (x > wr);
             const float th = left ? 0.f : (right ? 1.f : th0);
             const float kap = (left || right || flat) ? 0.f : (l0 + th0 * (l1 - l0)) * r;
-            const float w0 = lane_of(rw, li), w1 = lane_of(rw, li + 1);
-            const float u0 = lane_of(ru, li), u1 = lane_of(ru, li + 1);
-            const float s0 = lane_of(rs, li), s1 = lane_of(rs, li + 1);
+            const float w0 = lane_of(rv.w, li), w1 = lane_of(rv.w, li + 1);
+            const float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
+            const float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
             const float gk = -gj * kap, dk = -dj * kap;
             const float dw = w1 - w0;
             const float Rwv = w0 + th * dw;
@@ -1162,19 +1205,32 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         }
         if (gather && valid) bwd_bin_gather(g, R, s, B, h, x, gj, dj, v);
       };
-      uint32_t jb = j0;
-      for (; jb + 64 <= j1; jb += 64) group(jb, std::true_type{});
-      if (jb < j1) group(jb, std::false_type{});
-      };
-      if (interior) groups(std::true_type{});
-      else groups(std::false_type{});
+      // groups of the short path that must be redone (first 64 groups)
+      uint64_t redo = 0;
+      int n_fast = 0;
+      if (interior) {
+        for (uint32_t jb = j0; jb + 64 <= j1 && n_fast < 64; jb += 64, ++n_fast)
+          if (!fast_group(jb)) redo |= uint64_t(1) << n_fast;
+      }
+      int gi = 0;
+      for (uint32_t jb = j0; jb < j1; jb += 64, ++gi) {
+        if (gi < n_fast && !((redo >> gi) & 1)) continue;
+        const bool full = jb + 64 <= j1;
+        if (interior) {
+          if (full) group(jb, std::true_type{}, std::true_type{});
+          else group(jb, std::true_type{}, std::false_type{});
+        } else {
+          if (full) group(jb, std::false_type{}, std::true_type{});
+          else group(jb, std::false_type{}, std::false_type{});
+        }
+      }
     }
     const double tot = lds_sum<7>(v, red[t.wave], t.lane);
     const int q = t.lane >> 3;
     if (live && (t.lane & 7) == 0 && q < 7) {
-      const int64_t base = (t.r * kSlabs + t.slab) * 8;
-      if (q < 5) vpart[base + q] = float(tot);
-      else apart[base + q - 5] = tot;
+      const int64_t it = t.r * kSlabs + t.slab;
+      if (q < kVSums) vpart[it * kVSums + q] = float(tot);
+      else apart[it * kASums + q - kVSums] = tot;
     }
   }
 }
@@ -1202,8 +1258,8 @@ __global__ void __launch_bounds__(256) star_grad_kernel(SspView g, rbx_stars st,
     // slab sums in slab order: Vg_w, Vg_u, Vg_s, Vd_u, Vd_s | Ag_w, Ad_w
     double Pg_m = 0, Pg_u = 0, Pg_s = 0, Pd_u = 0, Pd_s = 0, Pg_a = 0, Pd_a = 0;
     for (int q = 0; q < kSlabs; ++q) {
-      const float* vp = vpart + (r * kSlabs + q) * 8;
-      const double* ap = apart + (r * kSlabs + q) * 8;
+      const float* vp = vpart + (r * kSlabs + q) * kVSums;
+      const double* ap = apart + (r * kSlabs + q) * kASums;
       Pg_m += vp[0];
       Pg_u += vp[1];
       Pg_s += vp[2];
@@ -1213,6 +1269,12 @@ __global__ void __launch_bounds__(256) star_grad_kernel(SspView g, rbx_stars st,
       Pd_a += ap[1];
     }
     const Cell cell = make_cell(g, st.metallicity[i], st.age[i]);
+    // the kernels' u and s sums are of (A + fs E) and (C + fu E): times the
+    // reciprocal cell sizes they are the sums of the corner derivatives
+    Pg_u *= double(cell.inv_dx);
+    Pd_u *= double(cell.inv_dx);
+    Pg_s *= double(cell.inv_dy);
+    Pd_s *= double(cell.inv_dy);
     double win[4];
     window_sums(g, s.row, s.kl, s.kh, win);
     const double m = s.m, a = s.a, sf = s.sf, T = s.T, Nn = s.Nn;
@@ -1307,10 +1369,10 @@ struct CubeLayout {
   StarState* state;
   ChunkDesc* desc;
   double* partial;
-  float* gradw;    // (n, 8) corner-weight derivatives (gu, gs) per sorted star
+  float* gradw;    // (n, 2) fractional cell position (fu, fs) per sorted star
   float* nn_part;  // (n, kSlabs) slab partials of Nn
-  float* vpart;    // (n, kSlabs, 8) adjoint value sums
-  double* apart;   // (n, kSlabs, 8) adjoint Doppler sums
+  float* vpart;    // (n, kSlabs, kVSums) adjoint value sums
+  double* apart;   // (n, kSlabs, kASums) adjoint Doppler sums
   void* sort_ws;
   size_t sort_ws_bytes;
   int64_t max_chunks;
@@ -1328,10 +1390,10 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, 
   auto* perm = c.template take<int32_t>(n1);
   auto* skey = c.template take<int32_t>(n1);
   auto* state = c.template take<StarState>(n1);
-  auto* gw = c.template take<float>(n1 * 8);
+  auto* gw = c.template take<float>(n1 * 2);
   auto* nnp = c.template take<float>(n1 * kSlabs);
-  auto* vp = c.template take<float>(n1 * kSlabs * 8);
-  auto* ap = c.template take<double>(n1 * kSlabs * 8);
+  auto* vp = c.template take<float>(n1 * kSlabs * kVSums);
+  auto* ap = c.template take<double>(n1 * kSlabs * kASums);
   if (per_star_only) {
     if (L) {
       L->perm = reinterpret_cast<int32_t*>(perm);

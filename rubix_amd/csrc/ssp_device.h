@@ -128,6 +128,12 @@ struct Cell {
   bool inside;
   float w[4], gu[4], gs[4];
   float du_dZ, ds_dt;
+  // fractional position in the cell, fu = (u - x1)/(x2 - x1), fs = (s - y1)/(y2 - y1),
+  // and the reciprocal cell sizes: with A = f10 - f00, C = f01 - f00 and
+  // E = f11 - f10 - f01 + f00 the corner sums are
+  //   sum_c w_c f_c  = f00 + fu A + fs (C + fu E),
+  //   sum_c gu_c f_c = (A + fs E) inv_dx,   sum_c gs_c f_c = (C + fu E) inv_dy.
+  float fu, fs, inv_dx, inv_dy;
 };
 
 __device__ inline Cell make_cell(const SspView& g, float Z, float t) {
@@ -151,6 +157,10 @@ __device__ inline Cell make_cell(const SspView& g, float Z, float t) {
   c.w[3] = ax1 * ay1 * inv;
   c.gu[0] = -ay2 * inv; c.gu[1] = ay2 * inv; c.gu[2] = -ay1 * inv; c.gu[3] = ay1 * inv;
   c.gs[0] = -ax2 * inv; c.gs[1] = -ax1 * inv; c.gs[2] = ax2 * inv; c.gs[3] = ax1 * inv;
+  c.fu = ax1 / (x2 - x1);
+  c.fs = ay1 / (y2 - y1);
+  c.inv_dx = 1.0f / (x2 - x1);
+  c.inv_dy = 1.0f / (y2 - y1);
   c.row = (iz - 1) * na + (ja - 1);
   c.du_dZ = g.log_axes ? 1.0f / (Z * kLn10) : 1.0f;
   c.ds_dt = g.log_axes ? 1.0f / (t * kLn10) : 1.0f;

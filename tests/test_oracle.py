@@ -292,3 +292,24 @@ def test_separable_psf_equals_direct(ny, nx, nl, m, sig):
     (a * torch.tensor(g)).sum().backward()
     (b * torch.tensor(g)).sum().backward()
     assert abs(float(sa.grad) - float(sb.grad)) <= 1e-12 * max(abs(float(sa.grad)), 1e-30)
+
+
+def test_lsf_even_tap_count():
+    """wave_res 1.6 (HECTOR) and 2.7 (CALIFA): the float arange of the kernel
+    grid has 26 taps; the full convolution cropped at 12 is n + 1 long (the
+    reference's reshape raises) and the oracle keeps its first n bins, the
+    x = 0 tap (index 12) on each input bin."""
+    for res in (1.6, 2.7):
+        k = R.lsf_kernel(1.0, res, dtype=np.float64)
+        assert len(k) == 26
+        assert abs(float(np.arange(-12 * res, 12 * res + res, res)[12])) < 1e-12
+        rng = np.random.default_rng(1)
+        spec = rng.normal(size=(3, 40))
+        full = np.stack([np.convolve(s, k, mode="full") for s in spec])
+        assert full[:, 12:40 + 26 - 1 - 12].shape[1] == 41
+        np.testing.assert_allclose(R.apply_lsf_kernel(spec, k), full[:, 12:52], rtol=1e-13,
+                                   atol=1e-14)
+        # a delta stays centred: the output peaks on the input bin
+        d = np.zeros((1, 41))
+        d[0, 20] = 1.0
+        assert int(np.argmax(R.apply_lsf_kernel(d, R.lsf_kernel(2 * res, res, dtype=np.float64)))) == 20
