@@ -718,7 +718,8 @@ def run_batched(args):
         eager_step(ops.Tuning())
         torch.cuda.synchronize()
         return
-    key = autotune.problem_key(L.n_total, L.n_rows, nt, n_src, dev)
+    key = autotune.problem_key(L.n_total, L.n_rows, nt, n_src, dev,
+                               n_templates=grid["flux"].shape[0] * grid["flux"].shape[1])
     t0 = time.perf_counter()
     if args.no_tune:
         best, seen = ops.Tuning(), {}

@@ -39,14 +39,17 @@ def cache_path() -> Path:
     return Path.home() / ".cache" / "rubix_amd" / "tuning.json"
 
 
-def problem_key(n_stars: int, n_rows: int, n_target: int, n_wave: int, device=None) -> str:
+def problem_key(n_stars: int, n_rows: int, n_target: int, n_wave: int, device=None,
+                n_templates: int = 0) -> str:
     """Cache key: device name and CU count, star and row counts bucketed to
-    powers of two, the two wavelength axis lengths."""
+    powers of two, the two wavelength axis lengths and the number of
+    templates of the grid (n_met x n_age: with n_wave it sets the template
+    bytes, which decide the number of age bands of the cube's sort keys)."""
     props = torch.cuda.get_device_properties(device)
     nb = max(0, int(math.log2(max(n_stars, 1))))
     rb = max(0, int(math.log2(max(n_rows, 1))))
     return (f"{props.name}|cu{props.multi_processor_count}|n2^{nb}|rows2^{rb}"
-            f"|nt{n_target}|nw{n_wave}")
+            f"|nt{n_target}|nw{n_wave}|tpl{n_templates}")
 
 
 def _load(path: Path) -> dict:
@@ -89,7 +92,7 @@ def autotune(run, key: str, reps: int = 3, use_cache: bool = True, log=None):
     data = _load(path) if use_cache else {}
     if use_cache and key in data:
         d = data[key]
-        return Tuning(d["chunk"], d["waves"], d["persist"]), {}
+        return Tuning(d["chunk"], d["waves"], d["persist"], d.get("flags", 0)), {}
     best = Tuning()
     seen = {best: time_ms(run, best, reps)}
     for name in ("waves", "persist", "chunk"):
