@@ -422,7 +422,9 @@ __device__ inline void win_values_interior(const float (&x)[NB], const int (&k0)
     li[t] = l;
     wl[t] = mul_nocontract(lam[l], h.a);
     wr[t] = mul_nocontract(lam[l + 1], h.a);
-    any = any || (x[t] >= wr[t] && l < li_max) || (x[t] < wl[t] && l > 0);
+    // l is in [2, nk - 4] here (the window's margins), inside both bounds of
+    // the correction step, so a step is needed exactly when x is off [wl, wr)
+    any = any || x[t] >= wr[t] || x[t] < wl[t];
   }
   if (__ballot(any)) {
 #pragma unroll
@@ -806,7 +808,9 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
       __syncthreads();  // every share of this star posted
       const float Nn = sum16_row0(nn_s[par][lane & 15]);
       par ^= 1;
-      float sf = s.T / Nn;
+      // T / Nn with the hardware reciprocal (within one ulp of the division;
+      // Nn = 0 still gives inf or nan for nan_to_num below)
+      float sf = s.T * __builtin_amdgcn_rcpf(Nn);
       if (isnan(sf)) sf = 0.f;
       else if (isinf(sf)) sf = sf > 0.f ? FLT_MAX : -FLT_MAX;
       if (wave == 0 && lane == 0) {
@@ -919,19 +923,24 @@ __device__ inline void bwd_bin_gather(const SspView& g, const Rows& R, const Sta
   Pair f[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) f[c] = ld_pair(R.f[c], K.i0);
+  // rows at the left knot, and their change to the right knot from the
+  // corners' own knot-to-knot differences (exact in float32 for neighbouring
+  // values within a factor 2 of each other), so that the Doppler sums, which
+  // cancel strongly across bins, are not left with the rounding of the rows
+  // themselves (it dominates where the knots are denser than the bins)
   const RowVals a = combine_rows(f[0].lo, f[1].lo, f[2].lo, f[3].lo, B.fu, B.fs);
-  const RowVals b = combine_rows(f[0].hi, f[1].hi, f[2].hi, f[3].hi, B.fu, B.fs);
-  const float dw = b.w - a.w;
-  const float Rw = a.w + K.theta * dw;
-  const float Ru = a.u + K.theta * (b.u - a.u);
-  const float Rs = a.s + K.theta * (b.s - a.s);
+  const RowVals d = combine_rows(f[0].hi - f[0].lo, f[1].hi - f[1].lo, f[2].hi - f[2].lo,
+                                 f[3].hi - f[3].lo, B.fu, B.fs);
+  const float Rw = a.w + K.theta * d.w;
+  const float Ru = a.u + K.theta * d.u;
+  const float Rs = a.s + K.theta * d.s;
   v[0] += gj * Rw;
   v[1] += gj * Ru;
   v[2] += gj * Rs;
   v[3] += dj * Ru;
   v[4] += dj * Rs;
-  v[5] += gk * dw;
-  v[6] += dk * dw;
+  v[5] += gk * d.w;
+  v[6] += dk * d.w;
 }
 
 // Slab-mapped like star_norm_kernel.  The per-star gradients need the four
