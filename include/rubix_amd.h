@@ -337,6 +337,20 @@ int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer,
                       int32_t n_taps, int32_t start, double* result,
                       void* workspace, size_t workspace_bytes, void* stream);
 
+/* rbx_conv_axis_dot without a workspace argument and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The block partials (at most 4096 doubles, 32 KiB, for any input size; a
+ * larger count is rejected with RBX_EINVAL) live in a file-scope device array
+ * that each device keeps until the process ends; its address is looked up
+ * on the first call on a device.  Every call on one device shares that
+ * array: calls that overlap on one device race (serialise them on one
+ * stream).  Same kernels and result as rbx_conv_axis_dot. */
+int rbx_conv_axis_dot_noalloc(const float* a, const float* b, int64_t n_outer,
+                              int32_t n_axis, int64_t n_inner, const double* taps,
+                              int32_t n_taps, int32_t start, double* result,
+                              void* stream);
+
 /* ---------------------------------------------------------------- alignment
  * The steps before spaxel assignment in the reference chain
  * (rubix/config/pipeline_config.yml: rotate_galaxy -> filter_particles).

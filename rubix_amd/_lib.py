@@ -141,6 +141,8 @@ SIGNATURES = {
     "rbx_dot_workspace_size": (c_size_t, [c_int64]),
     "rbx_conv_axis_dot": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_void_p, c_int32,
                                   c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "rbx_conv_axis_dot_noalloc": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_void_p,
+                                          c_int32, c_int32, c_void_p, c_void_p]),
 }
 
 _lib = None

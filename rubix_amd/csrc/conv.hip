@@ -21,6 +21,8 @@
 // The batched PSF (rbx_conv_batch) uses row items of the batch cube: one
 // spaxel row and a chunk of lambda, its galaxy found once per item.  Every
 // output sums taps in the same order t = 0 .. m-1 as the flat definition.
+#include <mutex>
+
 #include "rbx_common.h"
 
 namespace rbx {
@@ -700,6 +702,61 @@ int conv_axis(const float* in, float* out, int64_t n_outer, int n_axis, int64_t 
   return check_launch("rbx_conv_axis");
 }
 
+// dot = sum(a * conv_axis(b)) into result, the block partials in `partial`
+// (nblk doubles, every one written by its block before the final sum).
+static int conv_axis_dot(const float* a, const float* b, int64_t n_outer, int n_axis,
+                         int64_t n_inner, const double* taps, int n_taps, int start,
+                         double* partial, int nblk, double* result, hipStream_t s,
+                         const char* who) {
+  if (n_inner == 1 && n_taps - 1 <= kHalo) {
+    const int64_t n_seg = (n_axis + kSeg - 1) / kSeg;
+    hipLaunchKernelGGL(conv_dot_rows_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_axis,
+                       taps, n_taps, start, n_seg, n_outer * n_seg, partial);
+  } else if (n_inner > 1 && n_axis <= kTileAxis) {
+    const int64_t n_blocks = (n_inner + 63) / 64;
+    hipLaunchKernelGGL(conv_dot_tile_kernel, dim3(nblk), dim3(kDotThreads),
+                       size_t(n_axis) * 64 * sizeof(float), s, a, b, n_axis, n_inner, taps, n_taps,
+                       start, n_blocks, n_outer * n_blocks, partial);
+  } else if (n_inner > 1) {
+    const int64_t n_chunks = (n_inner + kLineChunk - 1) / kLineChunk;
+    hipLaunchKernelGGL(conv_dot_lines_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,
+                       n_axis, n_inner, taps, n_taps, start, n_chunks, n_outer * n_axis * n_chunks,
+                       partial);
+  } else {
+    hipLaunchKernelGGL(conv_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,
+                       n_axis, n_inner, taps, n_taps, start, partial);
+  }
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, nblk, result);
+  return check_launch(who);
+}
+
+// Block partials of rbx_conv_axis_dot_noalloc: a file-scope device array
+// instead of a caller's workspace, kDotBlocksMax doubles (32 KiB), which
+// every device the library runs on holds until the process ends.  Host code
+// uses its per-device address (scratch_ptr), never the array's name.
+static __device__ __align__(256) double scratch[kDotBlocksMax];
+
+// Device address of `scratch` on the current device, looked up with
+// hipGetSymbolAddress on the device's first call and kept per device.
+static int scratch_ptr(double** out, const char* who) {
+  static std::mutex mu;
+  static void* ptrs[64] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return failf(RBX_EHIP, "%s: no current device", who);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return failf(RBX_EHIP, "%s: hipGetSymbolAddress(scratch) failed: %s", who,
+                   hipGetErrorString(e));
+    ptrs[dev] = p;
+  }
+  *out = static_cast<double*>(ptrs[dev]);
+  return RBX_OK;
+}
+
 }  // namespace rbx
 
 extern "C" {
@@ -754,28 +811,28 @@ int rbx_conv_axis_dot(const float* a, const float* b, int64_t n_outer, int32_t n
   const int nblk = dot_blocks(total);
   if (!workspace || workspace_bytes < size_t(nblk) * sizeof(double))
     return fail(RBX_EWORKSPACE, "rbx_conv_axis_dot: workspace too small");
-  hipStream_t s = as_stream(stream);
-  double* partial = static_cast<double*>(workspace);
-  if (n_inner == 1 && n_taps - 1 <= kHalo) {
-    const int64_t n_seg = (n_axis + kSeg - 1) / kSeg;
-    hipLaunchKernelGGL(conv_dot_rows_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_axis,
-                       taps, n_taps, start, n_seg, n_outer * n_seg, partial);
-  } else if (n_inner > 1 && n_axis <= kTileAxis) {
-    const int64_t n_blocks = (n_inner + 63) / 64;
-    hipLaunchKernelGGL(conv_dot_tile_kernel, dim3(nblk), dim3(kDotThreads),
-                       size_t(n_axis) * 64 * sizeof(float), s, a, b, n_axis, n_inner, taps, n_taps,
-                       start, n_blocks, n_outer * n_blocks, partial);
-  } else if (n_inner > 1) {
-    const int64_t n_chunks = (n_inner + kLineChunk - 1) / kLineChunk;
-    hipLaunchKernelGGL(conv_dot_lines_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,
-                       n_axis, n_inner, taps, n_taps, start, n_chunks, n_outer * n_axis * n_chunks,
-                       partial);
-  } else {
-    hipLaunchKernelGGL(conv_dot_partial_kernel, dim3(nblk), dim3(kDotThreads), 0, s, a, b, n_outer,
-                       n_axis, n_inner, taps, n_taps, start, partial);
-  }
-  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kDotThreads), 0, s, partial, nblk, result);
-  return check_launch("rbx_conv_axis_dot");
+  return conv_axis_dot(a, b, n_outer, n_axis, n_inner, taps, n_taps, start,
+                       static_cast<double*>(workspace), nblk, result, as_stream(stream),
+                       "rbx_conv_axis_dot");
+}
+
+int rbx_conv_axis_dot_noalloc(const float* a, const float* b, int64_t n_outer, int32_t n_axis,
+                              int64_t n_inner, const double* taps, int32_t n_taps,
+                              int32_t start, double* result, void* stream) {
+  using namespace rbx;
+  const char* who = "rbx_conv_axis_dot_noalloc";
+  if (!a || !b || !taps || !result || n_outer < 0 || n_axis < 1 || n_inner < 1 || n_taps < 1)
+    return failf(RBX_EINVAL, "%s: bad arguments", who);
+  const int64_t total = n_outer * n_axis * n_inner;
+  const int nblk = dot_blocks(total);
+  if (nblk > kDotBlocksMax)
+    return failf(RBX_EINVAL, "%s: %d block partials exceed the %d-double scratch", who, nblk,
+                 kDotBlocksMax);
+  double* partial = nullptr;
+  const int rc = scratch_ptr(&partial, who);
+  if (rc) return rc;
+  return conv_axis_dot(a, b, n_outer, n_axis, n_inner, taps, n_taps, start, partial, nblk, result,
+                       as_stream(stream), who);
 }
 
 int rbx_conv_batch(const float* in, float* out, const rbx_galaxy* galaxies, int32_t n_gal,
