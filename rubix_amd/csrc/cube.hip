@@ -51,6 +51,7 @@ namespace rbx {
 
 constexpr int kTile = 64;    // target bins per wavefront tile
 constexpr int kSlabs = 8;    // lambda slabs, one per XCD
+constexpr int kBwdSlabs = 4; // lambda slabs of the adjoint's (star, slab) items
 
 struct __align__(16) StarState {
   int32_t row;      // flux row of corner (iz-1, ja-1); -1 = contributes nothing
@@ -483,23 +484,25 @@ __device__ inline void slab_window(const SspView& g, const rbx_target& tg, uint3
 }
 
 // Work items of the slab-mapped per-star kernels: item q is lambda slab
-// q % 8 of the W stars [(q / 8) W, (q / 8) W + W), one per wavefront of a
-// workgroup of W waves.  Workgroup b takes items b, b + G, b + 2G, ... with
-// G = gridDim.x a multiple of 8 (G = number of items, or a persistent grid
-// of a few workgroups per CU), so it always holds slab b % 8 and runs on
-// XCD b % 8: each XCD's L2 holds only its slab of the templates.
+// q % S of the W stars [(q / S) W, (q / S) W + W), one per wavefront of a
+// workgroup of W waves (S = 8 slabs for star_norm, 4 for the adjoint).
+// Workgroup b takes items b, b + G, b + 2G, ... with G = gridDim.x a
+// multiple of 8 (G = number of items, or a persistent grid of a few
+// workgroups per CU), so it always holds slab b % S and runs on XCD b % 8:
+// each XCD's L2 holds only one slab of the templates.
 struct SlabTask {
   int slab, wave, lane;
   int64_t r;
 };
 
-template <int W>
+template <int W, int S = kSlabs>
 __device__ inline SlabTask slab_task(int64_t q) {
+  static_assert((S & (S - 1)) == 0 && S <= 8, "slab count: a power of two dividing 8");
   SlabTask t;
-  t.slab = int(q & (kSlabs - 1));
+  t.slab = int(q & (S - 1));
   t.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   t.lane = threadIdx.x & 63;
-  t.r = (q >> 3) * W + t.wave;
+  t.r = (q / S) * W + t.wave;
   return t;
 }
 
@@ -685,7 +688,7 @@ constexpr int kMaxFusedBins = 8;
 // value is computed once instead of twice.  The first wavefront stores sf
 // and Nn for the adjoint.
 constexpr int kFusedWaves = 2 * kSlabs;
-static_assert(kFusedWaves == 16, "the Nn shares are added as one 16-lane row");
+static_assert(kFusedWaves == 16, "the Nn shares fill one 64-lane wave: 16 waves x 4 rows");
 
 // v + the row-of-16 DPP move of v (every lane of the wave active).
 template <int kCtrl>
@@ -693,16 +696,26 @@ __device__ inline float plus_dpp(float v) {
   return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kCtrl, 0xF, 0xF, false));
 }
 
-// Sum of the 16 values held by lanes 0..15 (lanes 16..63 hold copies of
-// them), the same fixed tree in every wavefront: pairs (xor 1), quads
-// (xor 2), then rotations of the row by 4 and 8; lane 0's total, broadcast.
-// All 64 lanes must be active.
-__device__ inline float sum16_row0(float v) {
+// Every lane gets the sum of its 16-lane row, by the same fixed tree in
+// every row: pairs (xor 1), quads (xor 2), then rotations of the row by 4
+// and 8.  All 64 lanes must be active.
+__device__ inline float row_sum16(float v) {
   v = plus_dpp<0xB1>(v);   // quad_perm(1, 0, 3, 2)
   v = plus_dpp<0x4E>(v);   // quad_perm(2, 3, 0, 1)
   v = plus_dpp<0x124>(v);  // row_ror:4
-  v = plus_dpp<0x128>(v);  // row_ror:8
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+  return plus_dpp<0x128>(v);  // row_ror:8
+}
+
+// Sum over the 64 lanes in a fixed order, broadcast: the four row sums of
+// row_sum16, added row 0 to row 3.  DPP moves and lane reads only (no LDS
+// round trips).  All 64 lanes must be active.
+__device__ inline float wave_sum_dpp(float v) {
+  v = row_sum16(v);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+  return ((r0 + r1) + r2) + r3;
 }
 // Knots of a wavefront's staged window: 80 per bin of a lane (the 64 NB
 // bins of a wavefront span ~64 NB x 1.25 A, ~57 NB knots of a 1.4 A grid,
@@ -723,7 +736,7 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
   constexpr int kHalfWindow = fused_window<NB>();
   __shared__ float lam_s[kFusedWaves][kHalfWindow];
   __shared__ float F_s[kFusedWaves][kHalfWindow];
-  __shared__ float nn_s[2][kFusedWaves];
+  __shared__ float nn_s[2][4 * kFusedWaves];
   const int n_chunks = *n_chunks_ptr;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int slab = wave >> 1, half = wave & 1;
@@ -803,10 +816,12 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
       float nn = 0.f;
 #pragma unroll
       for (int t = 0; t < kFusedBins; ++t) nn += P[t] * dt[t];
-      nn = wave_sum(nn);
-      if (lane == 0) nn_s[par][wave] = nn;
+      // four shares per wavefront (its row sums), 64 in all; after the
+      // barrier every wavefront adds the 64 in the same fixed order
+      nn = row_sum16(nn);
+      if ((lane & 15) == 0) nn_s[par][4 * wave + (lane >> 4)] = nn;
       __syncthreads();  // every share of this star posted
-      const float Nn = sum16_row0(nn_s[par][lane & 15]);
+      const float Nn = wave_sum_dpp(nn_s[par][lane]);
       par ^= 1;
       // T / Nn with the hardware reciprocal (within one ulp of the division;
       // Nn = 0 still gives inf or nan for nan_to_num below)
@@ -863,10 +878,20 @@ constexpr int kASums = 2;                // Doppler sums per (star, slab): Ag_w,
 constexpr int kRedRow = 68;              // LDS floats per value row (64 lanes, 16-byte rows)
 constexpr int kRedFloats = 8 * kRedRow;  // one wavefront's reduction buffer
 
+// The double of lane f(l) for a row DPP control (both halves moved).
+template <int kCtrl>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp(int(b & 0xffffffffll), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), kCtrl, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
 // v[0..K) of all 64 lanes -> lane 8q (q < K) holds the wave total of v[q]
 // (double).  Lane l stores v[q] at row q, column l; lane 8q + p adds the
 // row's columns 8p .. 8p + 7 in order (two 16-byte LDS reads), then the
-// eight parts are combined by xor-shuffles over 1, 2 and 4: a fixed order.
+// eight parts are combined by DPP moves over 1, 2 and 4 lanes: a fixed order.
 // buf is the wave's own LDS (kRedFloats floats, 16-byte aligned).
 template <int K>
 __device__ inline double lds_sum(const float v[K], float* buf, int lane) {
@@ -887,9 +912,9 @@ __device__ inline double lds_sum(const float v[K], float* buf, int lane) {
   acc += double(b.y);
   acc += double(b.z);
   acc += double(b.w);
-  acc += __shfl_xor(acc, 1, 64);
-  acc += __shfl_xor(acc, 2, 64);
-  acc += __shfl_xor(acc, 4, 64);
+  acc += dpp_f64<0xB1>(acc);   // lane ^ 1: quad_perm(1, 0, 3, 2)
+  acc += dpp_f64<0x4E>(acc);   // lane ^ 2: quad_perm(2, 3, 0, 1)
+  acc += dpp_f64<0x12C>(acc);  // lane + 4 within the row: row_ror:12
   return acc;
 }
 
@@ -964,7 +989,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
     const float* __restrict__ axav, float* __restrict__ vpart, double* __restrict__ apart) {
   __shared__ __align__(16) float red[W][kRedFloats];
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const SlabTask t = slab_task<W>(item);
+    const SlabTask t = slab_task<W, kBwdSlabs>(item);
     const bool live = t.r < n;
     float v[7];
 #pragma unroll
@@ -993,7 +1018,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_kernel(
     const double tot = lds_sum<7>(v, red[t.wave], t.lane);
     const int q = t.lane >> 3;
     if (live && (t.lane & 7) == 0 && q < 7) {
-      const int64_t it = t.r * kSlabs + t.slab;
+      const int64_t it = t.r * kBwdSlabs + t.slab;
       if (q < kVSums) vpart[it * kVSums + q] = float(tot);
       else apart[it * kASums + q - kVSums] = tot;
     }
@@ -1033,7 +1058,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
   __shared__ __align__(16) float red[W][kRedFloats];
   const int n_w = g.n_wave;
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const SlabTask t = slab_task<W>(item);
+    const SlabTask t = slab_task<W, kBwdSlabs>(item);
     const bool live = t.r < n;
     float v[7];
 #pragma unroll
@@ -1214,16 +1239,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         }
         if (gather && valid) bwd_bin_gather(g, R, s, B, h, x, gj, dj, v);
       };
-      // groups of the short path that must be redone (first 64 groups)
-      uint64_t redo = 0;
-      int n_fast = 0;
-      if (interior) {
-        for (uint32_t jb = j0; jb + 64 <= j1 && n_fast < 64; jb += 64, ++n_fast)
-          if (!fast_group(jb)) redo |= uint64_t(1) << n_fast;
-      }
-      int gi = 0;
-      for (uint32_t jb = j0; jb < j1; jb += 64, ++gi) {
-        if (gi < n_fast && !((redo >> gi) & 1)) continue;
+      auto general = [&](uint32_t jb) {
         const bool full = jb + 64 <= j1;
         if (interior) {
           if (full) group(jb, std::true_type{}, std::true_type{});
@@ -1232,12 +1248,27 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
           if (full) group(jb, std::false_type{}, std::true_type{});
           else group(jb, std::false_type{}, std::false_type{});
         }
+      };
+      // the short path over the first (up to 64) full groups of an interior
+      // slab, marking those it leaves; then the general path for the marked
+      // groups in order, and for every group after the short path's
+      uint64_t redo = 0;
+      uint32_t jb = j0;
+      if (interior) {
+        for (int gi = 0; gi < 64 && jb + 64 <= j1; ++gi, jb += 64)
+          if (!fast_group(jb)) redo |= uint64_t(1) << gi;
       }
+      while (redo) {
+        const int gi = __builtin_ctzll(redo);
+        redo &= redo - 1;
+        general(j0 + 64u * uint32_t(gi));
+      }
+      for (; jb < j1; jb += 64) general(jb);
     }
     const double tot = lds_sum<7>(v, red[t.wave], t.lane);
     const int q = t.lane >> 3;
     if (live && (t.lane & 7) == 0 && q < 7) {
-      const int64_t it = t.r * kSlabs + t.slab;
+      const int64_t it = t.r * kBwdSlabs + t.slab;
       if (q < kVSums) vpart[it * kVSums + q] = float(tot);
       else apart[it * kASums + q - kVSums] = tot;
     }
@@ -1266,9 +1297,9 @@ __global__ void __launch_bounds__(256) star_grad_kernel(SspView g, rbx_stars st,
     }
     // slab sums in slab order: Vg_w, Vg_u, Vg_s, Vd_u, Vd_s | Ag_w, Ad_w
     double Pg_m = 0, Pg_u = 0, Pg_s = 0, Pd_u = 0, Pd_s = 0, Pg_a = 0, Pd_a = 0;
-    for (int q = 0; q < kSlabs; ++q) {
-      const float* vp = vpart + (r * kSlabs + q) * kVSums;
-      const double* ap = apart + (r * kSlabs + q) * kASums;
+    for (int q = 0; q < kBwdSlabs; ++q) {
+      const float* vp = vpart + (r * kBwdSlabs + q) * kVSums;
+      const double* ap = apart + (r * kBwdSlabs + q) * kASums;
       Pg_m += vp[0];
       Pg_u += vp[1];
       Pg_s += vp[2];
@@ -1380,8 +1411,8 @@ struct CubeLayout {
   double* partial;
   float* gradw;    // (n, 2) fractional cell position (fu, fs) per sorted star
   float* nn_part;  // (n, kSlabs) slab partials of Nn
-  float* vpart;    // (n, kSlabs, kVSums) adjoint value sums
-  double* apart;   // (n, kSlabs, kASums) adjoint Doppler sums
+  float* vpart;    // (n, kBwdSlabs, kVSums) adjoint value sums
+  double* apart;   // (n, kBwdSlabs, kASums) adjoint Doppler sums
   void* sort_ws;
   size_t sort_ws_bytes;
   int64_t max_chunks;
@@ -1401,8 +1432,8 @@ static void cube_layout(C& c, int64_t n, int32_t n_keys, int32_t nt, int chunk, 
   auto* state = c.template take<StarState>(n1);
   auto* gw = c.template take<float>(n1 * 2);
   auto* nnp = c.template take<float>(n1 * kSlabs);
-  auto* vp = c.template take<float>(n1 * kSlabs * kVSums);
-  auto* ap = c.template take<double>(n1 * kSlabs * kASums);
+  auto* vp = c.template take<float>(n1 * kBwdSlabs * kVSums);
+  auto* ap = c.template take<double>(n1 * kBwdSlabs * kASums);
   if (per_star_only) {
     if (L) {
       L->perm = reinterpret_cast<int32_t*>(perm);
@@ -1720,8 +1751,8 @@ int cube_bwd_impl(const rbx_ssp* ssp, const rbx_target* target, const rbx_stars*
   const SspView g = view_of(*ssp);
   const int32_t nt = target->n;
   const int n_tiles = (nt + kTile - 1) / kTile;
-  const int tps = (n_tiles + kSlabs - 1) / kSlabs;
-  const int64_t items = int64_t(kSlabs) * ((n + t.waves - 1) / t.waves);
+  const int tps = (n_tiles + kBwdSlabs - 1) / kBwdSlabs;
+  const int64_t items = int64_t(kBwdSlabs) * ((n + t.waves - 1) / t.waves);
   const int64_t grid = item_grid(items, t.persist, 64 * t.waves);
   switch (t.waves) {
     case 1: launch_bwd<1>(grid, s, g, *target, n, items, L.state, L.gradw, dcube, tps * kTile, axav, L.vpart, L.apart); break;
