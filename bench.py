@@ -290,6 +290,10 @@ def timed_leg(step, warmup, steps):
     elapsed = time.perf_counter() - t0
     fwd = [a.elapsed_time(b) for a, b, _ in evs]
     adj = [b.elapsed_time(c) for _, b, c in evs]
+    # start-to-start spans of the steps on the GPU (the last: start to end);
+    # their sum against the wall time shows host-side stalls
+    timed_leg.spans = [evs[i][0].elapsed_time(evs[i + 1][0]) for i in range(steps - 1)] + \
+        [evs[-1][0].elapsed_time(evs[-1][2])]
     return elapsed, fwd, adj
 
 
@@ -465,7 +469,8 @@ def run_single(args, dust=False):
         roofline["step_effective_frac"] = round(step_frac, 4)
     split = {"fwd_ms_median": round(float(np.median(fwd_ms)), 4),
              "adj_ms_median": round(float(np.median(adj_ms)), 4),
-             "step_ms_median": round(float(np.median(np.add(fwd_ms, adj_ms))), 4)}
+             "step_ms_median": round(float(np.median(np.add(fwd_ms, adj_ms))), 4),
+             "step_spans_ms": [round(x, 3) for x in timed_leg.spans]}
 
     plain_res = None
     if args.plain_steps > 0:

@@ -402,47 +402,68 @@ __device__ inline WinKnots win_knots(const SspView& g, float x, const Shift& h, 
   return win_knots_k0<kKappa, kInterior>(g, x, h, k_lo, nk, lam, loc_k0(g, x, h));
 }
 
-// P_j of NB bins of a lane in an interior window (the conditions of
-// win_knots<false, true>), the bins interleaved: the same operations per
-// bin as win_knots_k0 followed by S0 + theta (S1 - S0) on the staged
-// S_k, so the same values, with the LDS reads of all bins in flight
-// together and one wave vote for the rare correction.
-template <int NB>
-__device__ inline void win_values_interior(const float (&x)[NB], const int (&k0)[NB],
-                                           const Shift& h, int k_lo, int nk, const float* lam,
-                                           const float* S, float (&P)[NB]) {
+// P_j of NB bins of a lane from a staged window of shifted knots W[k] =
+// w_{k_lo+k} and spectrum values S[k] = S_{k_lo+k}, the bins interleaved:
+// the locator guess k0, the next-knot test on the shifted knot, then the
+// exact correction (one wave vote for the rare step) to the segment with
+// w_l <= x < w_r, as jnp.interp's searchsorted, and S_l + theta (S_r - S_l).
+// The search ends on the same segment from any guess inside the window, so
+// the values are those of win_knots_k0 on the unshifted knots.
+// kInterior: the window lies strictly inside the grid (k_lo >= 1 and
+// k_lo + nk < n_wave) and holds every bin's guess with a margin of 3 knots
+// below and 5 above (the guesses' error bound), so no segment touches knot
+// 0 or n_wave - 1: the grid-end bounds, the flat-segment case and the
+// clamps cannot apply and are left out.
+template <int NB, bool kInterior>
+__device__ inline void win_values(const float (&x)[NB], const int (&k0)[NB], int k_lo, int nk,
+                                  int n_wave, const float* W, const float* S, float (&P)[NB]) {
   const int li_max = nk - 2;
   int li[NB];
   float wl[NB], wr[NB];
+  auto needs_step = [&](int l, float xv, float lo, float hi) {
+    if (kInterior) return (xv >= hi && l < li_max) | (xv < lo && l > 0);
+    return (xv >= hi && l < li_max && l + k_lo + 1 < n_wave - 1) |
+           (xv < lo && l > 0 && l + k_lo > 0);
+  };
   bool any = false;
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    const float y = x[t] * h.inv_a;
     int l = k0[t] - k_lo;
-    l += lam[l + 1] <= y ? 1 : 0;
+    if (!kInterior) l = l < 0 ? 0 : (l > li_max ? li_max : l);
+    l += W[l + 1] <= x[t] ? 1 : 0;
+    if (!kInterior) l = l > li_max ? li_max : l;
     li[t] = l;
-    wl[t] = mul_nocontract(lam[l], h.a);
-    wr[t] = mul_nocontract(lam[l + 1], h.a);
-    // l is in [2, nk - 4] here (the window's margins), inside both bounds of
-    // the correction step, so a step is needed exactly when x is off [wl, wr)
-    any = any || x[t] >= wr[t] || x[t] < wl[t];
+    wl[t] = W[l];
+    wr[t] = W[l + 1];
+    // interior: l lies in [3, nk - 5] (the window's margins), inside both
+    // bounds of the step, so a step is needed exactly when x is off [w_l, w_r)
+    if (kInterior)
+      any |= (x[t] >= wr[t]) | (x[t] < wl[t]);
+    else
+      any |= needs_step(l, x[t], wl[t], wr[t]);
   }
   if (__ballot(any)) {
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      bool step = (x[t] >= wr[t] && li[t] < li_max) || (x[t] < wl[t] && li[t] > 0);
-      while (step) {
+      while (needs_step(li[t], x[t], wl[t], wr[t])) {
         li[t] += x[t] >= wr[t] ? 1 : -1;
-        wl[t] = mul_nocontract(lam[li[t]], h.a);
-        wr[t] = mul_nocontract(lam[li[t] + 1], h.a);
-        step = (x[t] >= wr[t] && li[t] < li_max) || (x[t] < wl[t] && li[t] > 0);
+        wl[t] = W[li[t]];
+        wr[t] = W[li[t] + 1];
       }
     }
   }
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    const float r = __builtin_amdgcn_rcpf(wr[t] - wl[t]);
-    const float th = (x[t] - wl[t]) * r;
+    const float dx = wr[t] - wl[t];
+    const float r = __builtin_amdgcn_rcpf(dx);
+    float th = (x[t] - wl[t]) * r;
+    if (!kInterior) {
+      const int i = li[t] + k_lo + 1;  // global right knot
+      const bool flat = !(dx > 0.f);
+      const bool left = (i == 1) && (x[t] < wl[t]);
+      const bool right = (i == n_wave - 1) && (x[t] > wr[t]);
+      th = left ? 0.f : (right ? 1.f : (flat ? 0.f : th));
+    }
     const float S0 = S[li[t]], S1 = S[li[t] + 1];
     P[t] = S0 + th * (S1 - S0);
   }
@@ -677,16 +698,18 @@ constexpr int kMaxFusedBins = 8;
 // Normalisation and accumulation in one pass for ascending targets: item q
 // is spaxel chunk q, one workgroup of 16 wavefronts, two per lambda slab
 // (wavefront 2b + h takes the bins j0 + 64 NB h + lane + 64 t, t < NB, of
-// slab b; NB = 4 up to 4096 target bins).  Per star of the chunk each wavefront stages the knot window of its
-// own 256 bins in its LDS (no other wavefront reads it) and evaluates P_j on
-// them once (the per-bin arithmetic of star_norm_kernel and
-// cube_accum_kernel), then posts its share of Nn = sum_j P_j dtarget_j;
-// after one barrier every wavefront adds the 16 shares in a fixed order,
-// forms sf = nan_to_num(T / Nn) and accumulates P_j sf into its f64
-// registers.  The shares are double-buffered, so a wavefront may post the
-// next star's share while another still reads this one's.  Each (star, bin)
-// value is computed once instead of twice.  The first wavefront stores sf
-// and Nn for the adjoint.
+// slab b; NB = 4 up to 4096 target bins).  The chunk's star states are
+// staged in LDS, 128 at a time.  Per star each wavefront takes its knot
+// window from the locator guesses of its first and last bin, stages the
+// window's shifted knots and spectrum values in its own LDS (no other
+// wavefront reads them) and evaluates P_j on its bins once (the per-bin
+// arithmetic of star_norm_kernel and cube_accum_kernel), then posts four
+// shares of Nn = sum_j P_j dtarget_j; after one barrier every wavefront adds
+// the 64 shares in a fixed order, forms sf = nan_to_num(T / Nn) and
+// accumulates P_j sf into its f64 registers.  The shares are
+// double-buffered, so a wavefront may post the next star's share while
+// another still reads this one's.  Each (star, bin) value is computed once
+// instead of twice.  The first wavefront stores sf and Nn for the adjoint.
 constexpr int kFusedWaves = 2 * kSlabs;
 static_assert(kFusedWaves == 16, "the Nn shares fill one 64-lane wave: 16 waves x 4 rows");
 
@@ -723,6 +746,15 @@ __device__ inline float wave_sum_dpp(float v) {
 template <int NB>
 constexpr int fused_window() { return 80 * NB; }
 
+// Star states staged in LDS per batch: the per-star fields come from LDS
+// (broadcast reads) instead of a global round trip ahead of every star.
+constexpr int kStateBatch = 64;
+
+// A wave-uniform float (the lanes hold the same value) in a scalar register.
+__device__ inline float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // Up to 4 bins per lane the kernel holds to 64 VGPRs (8 wavefronts per
 // SIMD); more bins per lane take 128 (4 per SIMD).
 template <bool kDust, int NB>
@@ -732,115 +764,155 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
                   const ChunkDesc* __restrict__ desc, const int32_t* __restrict__ n_chunks_ptr,
                   int slab_bins, int64_t n_items, const float* __restrict__ axav,
                   double* __restrict__ partial) {
-  constexpr int kFusedBins = NB;
   constexpr int kHalfWindow = fused_window<NB>();
-  __shared__ float lam_s[kFusedWaves][kHalfWindow];
-  __shared__ float F_s[kFusedWaves][kHalfWindow];
+  __shared__ float W_s[kFusedWaves][kHalfWindow];  // shifted knots of each wavefront's window
+  __shared__ float S_s[kFusedWaves][kHalfWindow];  // the star's spectrum at those knots
+  __shared__ float4 st_s[4 * kStateBatch];
   __shared__ float nn_s[2][4 * kFusedWaves];
+  __shared__ float x_s[kFusedWaves][NB][64];   // the lanes' target wavelengths
+  __shared__ float dt_s[kFusedWaves][NB][64];  // and bin widths
+  static_assert(sizeof(StarState) == 4 * sizeof(float4), "a star state is four float4");
   const int n_chunks = *n_chunks_ptr;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int slab = wave >> 1, half = wave & 1;
   const int lane = threadIdx.x & 63;
-  float* lam = lam_s[wave];
-  float* F = F_s[wave];
+  float* W = W_s[wave];
+  float* S = S_s[wave];
   const uint32_t j0 = uint32_t(slab * slab_bins);
   const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
   // this wavefront's bins [jh, jh_end)
-  const uint32_t jh = j0 + 64u * kFusedBins * half;
-  const uint32_t jh_end = jh + 64u * kFusedBins < j1 ? jh + 64u * kFusedBins : j1;
+  const uint32_t jh = j0 + 64u * NB * half;
+  const uint32_t jh_end = jh + 64u * NB < j1 ? jh + 64u * NB : j1;
+  // the lane's bins, the same for every chunk: a bin past the end takes the
+  // last bin's x (inside the window) and weight 0, so it adds nothing to Nn
+  // and its sum is never stored.  x and the widths wait in LDS and are read
+  // where used (fewer registers live through the staging loop).
+  float tau[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const uint32_t j = jh + lane + 64u * t;
+    const uint32_t jc = j < j1 ? j : (j1 > j0 ? j1 - 1 : 0);
+    const Pair xd = ld_pair(tg.pairs, 2 * jc);
+    x_s[wave][t][lane] = xd.lo;
+    dt_s[wave][t][lane] = j < jh_end ? xd.hi : 0.f;
+    tau[t] = kDust ? dust_tau(axav, jc) : 0.f;
+  }
   for (int64_t q = blockIdx.x; q < n_items; q += gridDim.x) {
     if (q >= n_chunks) break;  // chunk count is uniform: the whole workgroup leaves
     const ChunkDesc d = desc[q];
-    float x[kFusedBins], dt[kFusedBins], tau[kFusedBins];
-    double acc[kFusedBins];
+    double acc[NB];
 #pragma unroll
-    for (int t = 0; t < kFusedBins; ++t) {
-      const uint32_t j = jh + lane + 64u * t;
-      const uint32_t jc = j < j1 ? j : (j1 > j0 ? j1 - 1 : 0);
-      const Pair xd = ld_pair(tg.pairs, 2 * jc);
-      // a bin past the end takes the last bin's x (inside the window) and
-      // weight 0, so it adds nothing to Nn and its sum is never stored
-      x[t] = xd.lo;
-      dt[t] = j < jh_end ? xd.hi : 0.f;
-      tau[t] = kDust ? dust_tau(axav, jc) : 0.f;
-      acc[t] = 0.0;
-    }
+    for (int t = 0; t < NB; ++t) acc[t] = 0.0;
     int par = 0;
-    for (int r = d.begin; r < d.end; ++r) {
-      const StarState s = state[r];
-      if (s.row < 0) continue;  // the same star for every wavefront: uniform
-      float P[kFusedBins];
+    for (int b0 = d.begin; b0 < d.end; b0 += kStateBatch) {
+      const int nb_st = d.end - b0 < kStateBatch ? d.end - b0 : kStateBatch;
+      __syncthreads();  // the previous batch's states are read
+      const float4* src = reinterpret_cast<const float4*>(state + b0);
+      for (int i = threadIdx.x; i < 4 * nb_st; i += 64 * kFusedWaves) st_s[i] = src[i];
+      __syncthreads();
+      for (int i = 0; i < nb_st; ++i) {
+        const int r = b0 + i;
+        const float4 s0 = st_s[4 * i], s1 = st_s[4 * i + 1], s2 = st_s[4 * i + 2];
+        const int row = __builtin_amdgcn_readfirstlane(__float_as_int(s0.x));
+        if (row < 0) continue;  // the same star for every wavefront: uniform
+        StarState s;
+        s.row = row;
+        s.a = uniform_f(s0.z);
+        s.inv_a = uniform_f(s0.w);
+        s.w[0] = uniform_f(s1.x);
+        s.w[1] = uniform_f(s1.y);
+        s.w[2] = uniform_f(s1.z);
+        s.w[3] = uniform_f(s1.w);
+        s.m = uniform_f(s2.x);
+        s.T = uniform_f(s2.y);
+        s.av = kDust ? uniform_f(st_s[4 * i + 3].w) : 0.f;
+        float P[NB];
 #pragma unroll
-      for (int t = 0; t < kFusedBins; ++t) P[t] = 0.f;
-      if (jh < jh_end) {
-        const Rows R = rows_of(g, s.row);
-        const Shift h = shift_of(g, s.a, s.inv_a);
-        int k_lo, nk;
-        slab_window(g, tg, jh, jh_end, h, &k_lo, &nk);
-        if (nk <= kHalfWindow) {
-          // locator guesses first: their loads overlap the staging
-          int k0[kFusedBins];
+        for (int t = 0; t < NB; ++t) P[t] = 0.f;
+        if (jh < jh_end) {
+          const Rows R = rows_of(g, s.row);
+          const Shift h = shift_of(g, s.a, s.inv_a);
+          int k0[NB];
 #pragma unroll
-          for (int t = 0; t < kFusedBins; ++t) k0[t] = loc_k0(g, x[t], h);
-          for (int k = lane; k < nk; k += 64) {
-            const uint32_t kk = uint32_t(k_lo + k);
-            lam[k] = ld_at(g.wave, kk);
-            float f = 0.f;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
-            F[k] = f * s.m;  // S_k = F_k m, rounded once per knot as the reference's spectra
+          for (int t = 0; t < NB; ++t) k0[t] = loc_k0(g, x_s[wave][t][lane], h);
+          int k_lo, nk;
+          if (!g.loc_multi) {
+            // the guesses are monotone in x: lane 0's first and lane 63's
+            // last bin bound the wavefront's, widened by the error bound
+            const int lo = __builtin_amdgcn_readfirstlane(k0[0]) - 3;
+            const int hi = __builtin_amdgcn_readlane(k0[NB - 1], 63) + 5;
+            k_lo = lo < 0 ? 0 : lo;
+            nk = (hi > g.n_wave - 1 ? g.n_wave - 1 : hi) - k_lo + 1;
+          } else {
+            slab_window(g, tg, jh, jh_end, h, &k_lo, &nk);
           }
-          wave_lds_sync();
-          // every bin of the lane, also one past the end (its x lies in the window)
-          if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi) {
-            win_values_interior<kFusedBins>(x, k0, h, k_lo, nk, lam, F, P);
+          if (nk <= kHalfWindow) {
+            for (int k = lane; k < nk; k += 64) {
+              const uint32_t kk = uint32_t(k_lo + k);
+              const float lw = ld_at(g.wave, kk);
+              float fc[4];
+#pragma unroll
+              for (int c = 0; c < 4; ++c) fc[c] = ld_at(R.f[c], kk);
+              // the five loads in flight together, then the arithmetic
+              __builtin_amdgcn_sched_group_barrier(0x020, 5, 0);
+              __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
+              float f = 0.f;
+#pragma unroll
+              for (int c = 0; c < 4; ++c) f += s.w[c] * fc[c];
+              // shifted knot as the reference's wave * a; S_k = F_k m,
+              // rounded once per knot as the reference's spectra
+              W[k] = mul_nocontract(lw, s.a);
+              S[k] = f * s.m;
+            }
+            wave_lds_sync();
+            float x[NB];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) x[t] = x_s[wave][t][lane];
+            // every bin of the lane, also one past the end (its x lies in the window)
+            if (k_lo >= 1 && k_lo + nk < g.n_wave && !g.loc_multi)
+              win_values<NB, true>(x, k0, k_lo, nk, g.n_wave, W, S, P);
+            else
+              win_values<NB, false>(x, k0, k_lo, nk, g.n_wave, W, S, P);
+            wave_lds_sync();  // this window's reads done before the next star's stores
           } else {
 #pragma unroll
-            for (int t = 0; t < kFusedBins; ++t) {
-              const WinKnots K = win_knots_k0<false>(g, x[t], h, k_lo, nk, lam, k0[t]);
-              const float S0 = F[K.li], S1 = F[K.li + 1];
-              P[t] = S0 + K.theta * (S1 - S0);
-            }
-          }
-          wave_lds_sync();  // this window's reads done before the next star's stores
-        } else {
-#pragma unroll
-          for (int t = 0; t < kFusedBins; ++t) {
-            if (jh + lane + 64u * t < jh_end) {
-              const Knots K = knots_of<false>(g, x[t], h);
-              P[t] = star_value(R, s, K);
+            for (int t = 0; t < NB; ++t) {
+              if (jh + lane + 64u * t < jh_end) {
+                const Knots K = knots_of<false>(g, x_s[wave][t][lane], h);
+                P[t] = star_value(R, s, K);
+              }
             }
           }
         }
-      }
-      float nn = 0.f;
+        float nn = 0.f;
 #pragma unroll
-      for (int t = 0; t < kFusedBins; ++t) nn += P[t] * dt[t];
-      // four shares per wavefront (its row sums), 64 in all; after the
-      // barrier every wavefront adds the 64 in the same fixed order
-      nn = row_sum16(nn);
-      if ((lane & 15) == 0) nn_s[par][4 * wave + (lane >> 4)] = nn;
-      __syncthreads();  // every share of this star posted
-      const float Nn = wave_sum_dpp(nn_s[par][lane]);
-      par ^= 1;
-      // T / Nn with the hardware reciprocal (within one ulp of the division;
-      // Nn = 0 still gives inf or nan for nan_to_num below)
-      float sf = s.T * __builtin_amdgcn_rcpf(Nn);
-      if (isnan(sf)) sf = 0.f;
-      else if (isinf(sf)) sf = sf > 0.f ? FLT_MAX : -FLT_MAX;
-      if (wave == 0 && lane == 0) {
-        state[r].sf = sf;
-        state[r].Nn = Nn;
-      }
+        for (int t = 0; t < NB; ++t) nn += P[t] * dt_s[wave][t][lane];
+        // four shares per wavefront (its row sums), 64 in all; after the
+        // barrier every wavefront adds the 64 in the same fixed order
+        nn = row_sum16(nn);
+        if ((lane & 15) == 0) nn_s[par][4 * wave + (lane >> 4)] = nn;
+        __syncthreads();  // every share of this star posted
+        const float Nn = wave_sum_dpp(nn_s[par][lane]);
+        par ^= 1;
+        // T / Nn with the hardware reciprocal (within one ulp of the division;
+        // Nn = 0 still gives inf or nan for nan_to_num below)
+        float sf = s.T * __builtin_amdgcn_rcpf(Nn);
+        if (isnan(sf)) sf = 0.f;
+        else if (isinf(sf)) sf = sf > 0.f ? FLT_MAX : -FLT_MAX;
+        if (wave == 0 && lane == 0) {
+          state[r].sf = sf;
+          state[r].Nn = Nn;
+        }
 #pragma unroll
-      for (int t = 0; t < kFusedBins; ++t) {
-        float v = P[t] * sf;
-        if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
-        acc[t] += double(v);
+        for (int t = 0; t < NB; ++t) {
+          float v = P[t] * sf;
+          if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
+          acc[t] += double(v);
+        }
       }
     }
 #pragma unroll
-    for (int t = 0; t < kFusedBins; ++t) {
+    for (int t = 0; t < NB; ++t) {
       const uint32_t j = jh + lane + 64u * t;
       if (j < jh_end) partial[q * tg.n + j] = acc[t];
     }
