@@ -17,6 +17,13 @@
 //               gives each tile its base position inside each bucket.
 //   3. scatter: the same walk; position = base + rank inside the round's
 //               equal-key group, so stars keep their input order (stable).
+//   The (bucket, tile) table of 4096-star tiles is kept to 2^24 entries.  Key
+//   counts past that (a batch of many galaxies: 310 000 keys over 31.8 M
+//   stars) sort in LSD radix passes over D-bit digits of the bucket index,
+//   each pass the same stable count + scan + scatter over 2^D buckets, the
+//   scatter carrying each star's input index and bucket along; stable
+//   passes give the same order as the one-pass sort.  Bucket offsets then
+//   come from a binary search of the sorted buckets.
 #include <stdarg.h>
 #include <string.h>
 
@@ -155,36 +162,58 @@ struct SortPlan {
   int32_t buckets;  // n_keys + 1 (overflow bucket last)
   int64_t tile;     // stars per tile, multiple of 64
   int64_t ntiles;
-  int64_t table;    // buckets * ntiles
+  int64_t table;    // (buckets or digit buckets) * ntiles
   int64_t nblk;     // scan blocks over the table
   int nbits;        // bits needed to tell buckets apart
+  int digit_bits;   // 0: one pass over all buckets; else radix digits of this width
+  int passes;       // counting-sort passes
 };
+
+constexpr int64_t kTableCap = int64_t(1) << 24;  // (bucket, tile) counters
+constexpr int64_t kSortTile = 4096;              // stars per tile
 
 static SortPlan make_plan(int64_t n, int32_t n_keys) {
   SortPlan p;
   p.n = n;
   p.n_keys = n_keys;
   p.buckets = n_keys + 1;
-  p.tile = 4096;
-  const int64_t cap = int64_t(1) << 24;
-  while (p.tile < (int64_t(1) << 24) &&
-         int64_t(p.buckets) * ((n + p.tile - 1) / p.tile) > cap)
-    p.tile *= 2;
+  p.tile = kSortTile;
   p.ntiles = n > 0 ? (n + p.tile - 1) / p.tile : 1;
-  p.table = int64_t(p.buckets) * p.ntiles;
-  p.nblk = (p.table + kScanBlock - 1) / kScanBlock;
   int b = 1;
   while ((int64_t(1) << b) < p.buckets) ++b;
   p.nbits = b;
+  if (int64_t(p.buckets) * p.ntiles <= kTableCap) {
+    p.digit_bits = 0;
+    p.passes = 1;
+    p.table = int64_t(p.buckets) * p.ntiles;
+  } else {
+    // ntiles <= 2^19 for n < 2^31, so digits of at least 5 bits fit
+    int d = 16;
+    while (d > 1 && (int64_t(1) << d) * p.ntiles > kTableCap) --d;
+    p.digit_bits = d;
+    p.passes = (p.nbits + d - 1) / d;
+    p.table = (int64_t(1) << d) * p.ntiles;
+  }
+  p.nblk = (p.table + kScanBlock - 1) / kScanBlock;
   return p;
 }
 
+// Radix passes keep the input index and bucket of every position: three
+// n-element buffers (one index, two buckets) beside the caller's perm.
 template <class S>
-static void sort_layout(S& c, const SortPlan& p, int32_t** table, int32_t** blocks) {
+static void sort_layout(S& c, const SortPlan& p, int32_t** table, int32_t** blocks,
+                        int32_t** tperm, int32_t** b0, int32_t** b1) {
   auto* t = c.template take<int32_t>(p.table);
-  auto* b = c.template take<int32_t>(p.nblk + 1);
+  auto* bl = c.template take<int32_t>(p.nblk + 1);
+  const int64_t rn = p.digit_bits ? (p.n > 0 ? p.n : 1) : 0;
+  auto* tp = c.template take<int32_t>(rn);
+  auto* x0 = c.template take<int32_t>(rn);
+  auto* x1 = c.template take<int32_t>(rn);
   if (table) *table = reinterpret_cast<int32_t*>(t);
-  if (blocks) *blocks = reinterpret_cast<int32_t*>(b);
+  if (blocks) *blocks = reinterpret_cast<int32_t*>(bl);
+  if (tperm) *tperm = reinterpret_cast<int32_t*>(tp);
+  if (b0) *b0 = reinterpret_cast<int32_t*>(x0);
+  if (b1) *b1 = reinterpret_cast<int32_t*>(x1);
 }
 
 struct SizerAdapter {
@@ -211,11 +240,18 @@ __device__ inline uint64_t peer_mask(int key, uint64_t active, int nbits) {
   return peers;
 }
 
+// One counting-sort pass.  The sort key of element i is its bucket
+// bucket_of(in[i]), or with mask != 0 the digit (bucket >> shift) & mask.
+// Scatter: perm[pos] = the element's input index (src_perm[i], or i on the
+// first pass), bucket_out[pos] = its bucket, sorted[pos] = its original id
+// (ids[perm[pos]]; in[i] itself on the first pass).
 template <bool kScatter>
 __global__ void __launch_bounds__(64) sort_walk_kernel(
-    const int32_t* __restrict__ ids, int64_t n, int32_t n_keys, int64_t tile,
-    int64_t ntiles, int nbits, int32_t* __restrict__ table,
-    int32_t* __restrict__ perm, int32_t* __restrict__ sorted) {
+    const int32_t* __restrict__ in, int64_t n, int32_t n_keys, int64_t tile,
+    int64_t ntiles, int nbits, int shift, int32_t mask, int32_t* __restrict__ table,
+    const int32_t* __restrict__ src_perm, const int32_t* __restrict__ ids,
+    int32_t* __restrict__ perm, int32_t* __restrict__ bucket_out,
+    int32_t* __restrict__ sorted) {
   // blockDim.x == 64: one wavefront walks one tile in input order.
   const int lane = threadIdx.x;
   const int64_t t = blockIdx.x;
@@ -225,8 +261,9 @@ __global__ void __launch_bounds__(64) sort_walk_kernel(
   for (int64_t r = begin; r < end; r += 64) {
     const int64_t i = r + lane;
     const bool valid = i < end;
-    const int32_t id = valid ? ids[i] : 0;
-    const int key = bucket_of(id, n_keys);
+    const int32_t id = valid ? in[i] : 0;
+    const int full = bucket_of(id, n_keys);
+    const int key = mask ? ((full >> shift) & mask) : full;
     const uint64_t active = __ballot(valid);
     const uint64_t peers = peer_mask(key, active, nbits);  // all lanes take part
     const int leader = __ffsll((unsigned long long)peers) - 1;
@@ -239,10 +276,26 @@ __global__ void __launch_bounds__(64) sort_walk_kernel(
       base = __shfl(base, leader < 0 ? 0 : leader, 64);
       if (valid) {
         const int32_t pos = base + rank;
-        perm[pos] = int32_t(i);
-        if (sorted) sorted[pos] = id;
+        const int32_t src = src_perm ? src_perm[i] : int32_t(i);
+        perm[pos] = src;
+        if (bucket_out) bucket_out[pos] = full;
+        if (sorted) sorted[pos] = src_perm ? ids[src] : id;
       }
     }
+  }
+}
+
+// offsets[k] = number of sorted buckets below k, k in [0, n_keys].
+__global__ void bucket_offsets_kernel(const int32_t* __restrict__ bucket, int64_t n,
+                                      int32_t n_keys, int32_t* __restrict__ offsets) {
+  for (int64_t k = gtid(); k <= n_keys; k += int64_t(gridDim.x) * blockDim.x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (bucket[mid] < k) lo = mid + 1;
+      else hi = mid;
+    }
+    offsets[k] = int32_t(lo);
   }
 }
 
@@ -276,26 +329,44 @@ void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
 size_t sort_workspace_size(int64_t n, int32_t n_keys) {
   SortPlan p = make_plan(n, n_keys);
   SizerAdapter a;
-  sort_layout(a, p, nullptr, nullptr);
+  sort_layout(a, p, nullptr, nullptr, nullptr, nullptr, nullptr);
   return a.s.used;
 }
 
-// Workspace enough for any key count in [1, max_keys].  The count table is
-// (keys + 1) x tiles; tiles of 4096 stars double while the table would pass
-// 2^24 entries (up to tiles of 2^24 stars).  Below the first doubling the
-// table grows with the key count; past it, it stays under 2^24 entries
-// until the tile size is capped, and then grows again with the key count.
-// So max(table at max_keys, 2^24 entries) bounds every key count.
+// Workspace enough for any key count in [1, max_keys]: the table never
+// passes 2^24 entries, and the radix buffers are there whenever max_keys
+// would need them.
 size_t sort_workspace_bound(int64_t n, int32_t max_keys) {
   SortPlan p = make_plan(n, max_keys);
-  const int64_t tiles = n > 0 ? (n + 4095) / 4096 : 1;
-  if ((int64_t(max_keys) + 1) * tiles > (int64_t(1) << 24) && p.table < (int64_t(1) << 24)) {
-    p.table = int64_t(1) << 24;
-    p.nblk = (p.table + kScanBlock - 1) / kScanBlock;
-  }
+  p.table = kTableCap;
+  p.nblk = (p.table + kScanBlock - 1) / kScanBlock;
   SizerAdapter a;
-  sort_layout(a, p, nullptr, nullptr);
+  sort_layout(a, p, nullptr, nullptr, nullptr, nullptr, nullptr);
   return a.s.used;
+}
+
+// Count + scan of one pass; returns the launch status.
+static int sort_pass(const SortPlan& p, const int32_t* in, int shift, int32_t mask, int nbits,
+                     int32_t* table, int32_t* blocks, int64_t table_n, const int32_t* src_perm,
+                     const int32_t* ids, int32_t* perm, int32_t* bucket_out, int32_t* sorted,
+                     int32_t* offsets, hipStream_t s) {
+  fill_i32(table, table_n, 0, s);
+  if (p.n > 0) {
+    hipLaunchKernelGGL(sort_walk_kernel<false>, dim3(p.ntiles), dim3(64), 0, s, in, p.n,
+                       p.n_keys, p.tile, p.ntiles, nbits, shift, mask, table, nullptr, nullptr,
+                       nullptr, nullptr, nullptr);
+  }
+  int rc = exclusive_scan_i32(table, table_n, blocks, s);
+  if (rc) return rc;
+  if (offsets)
+    hipLaunchKernelGGL(sort_offsets_kernel, dim3(grid_for(int64_t(p.n_keys) + 1, 256)), dim3(256),
+                       0, s, table, p.n_keys, p.ntiles, offsets);
+  if (p.n > 0) {
+    hipLaunchKernelGGL(sort_walk_kernel<true>, dim3(p.ntiles), dim3(64), 0, s, in, p.n, p.n_keys,
+                       p.tile, p.ntiles, nbits, shift, mask, table, src_perm, ids, perm,
+                       bucket_out, sorted);
+  }
+  return check_launch("rbx_spaxel_sort");
 }
 
 // Sort used by the fused path too; the workspace is carved by the caller.
@@ -304,22 +375,31 @@ int spaxel_sort_impl(const int32_t* ids, int64_t n, int32_t n_keys, int32_t* per
                      hipStream_t s) {
   SortPlan p = make_plan(n, n_keys);
   Carver c(ws, ws_bytes);
-  int32_t *table = nullptr, *blocks = nullptr;
-  sort_layout(c, p, &table, &blocks);
+  int32_t *table = nullptr, *blocks = nullptr, *tperm = nullptr, *b0 = nullptr, *b1 = nullptr;
+  sort_layout(c, p, &table, &blocks, &tperm, &b0, &b1);
   if (!c.ok) return fail(RBX_EWORKSPACE, "rbx_spaxel_sort: workspace too small");
-  fill_i32(table, p.table, 0, s);
-  if (n > 0) {
-    hipLaunchKernelGGL(sort_walk_kernel<false>, dim3(p.ntiles), dim3(64), 0, s, ids, n, n_keys,
-                       p.tile, p.ntiles, p.nbits, table, nullptr, nullptr);
+  if (!p.digit_bits)
+    return sort_pass(p, ids, 0, 0, p.nbits, table, blocks, p.table, nullptr, ids, perm, nullptr,
+                     sorted, offsets, s);
+  // LSD radix: pass q sorts by bits [q D, q D + D) of the bucket, stably;
+  // the index buffers alternate so that the last pass writes perm
+  const int32_t mask = (int32_t(1) << p.digit_bits) - 1;
+  const int32_t* in = ids;
+  const int32_t* src = nullptr;
+  int32_t* bufs[2] = {b0, b1};
+  for (int q = 0; q < p.passes; ++q) {
+    const bool last = q == p.passes - 1;
+    int32_t* pout = ((p.passes - 1 - q) & 1) ? tperm : perm;
+    int32_t* bout = bufs[q & 1];
+    const int width = last ? p.nbits - q * p.digit_bits : p.digit_bits;
+    int rc = sort_pass(p, in, q * p.digit_bits, mask, width, table, blocks, p.table, src, ids,
+                       pout, bout, last ? sorted : nullptr, nullptr, s);
+    if (rc) return rc;
+    in = bout;
+    src = pout;
   }
-  int rc = exclusive_scan_i32(table, p.table, blocks, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(sort_offsets_kernel, dim3(grid_for(int64_t(n_keys) + 1, 256)), dim3(256), 0, s,
-                     table, n_keys, p.ntiles, offsets);
-  if (n > 0) {
-    hipLaunchKernelGGL(sort_walk_kernel<true>, dim3(p.ntiles), dim3(64), 0, s, ids, n, n_keys,
-                       p.tile, p.ntiles, p.nbits, table, perm, sorted);
-  }
+  hipLaunchKernelGGL(bucket_offsets_kernel, dim3(grid_for(int64_t(n_keys) + 1, 256)), dim3(256), 0,
+                     s, in, n, n_keys, offsets);
   return check_launch("rbx_spaxel_sort");
 }
 

@@ -99,6 +99,12 @@ def test_workspace_queries():
     assert 0 < a < b
     assert L.rbx_cube_workspace_size(-1, 100, 50) == 0
     assert L.rbx_spaxel_sort_workspace_size(10, 4) > 0
+    # one counting pass while (keys + 1) x (4096-star tiles) fits 2^24
+    # counters; past that, radix passes keep three n-element index buffers
+    one = L.rbx_spaxel_sort_workspace_size(10 ** 6, 2500)
+    radix = L.rbx_spaxel_sort_workspace_size(10 ** 6, 100_000)
+    assert one < 4 * 2 ** 22 and radix >= 12 * 10 ** 6
+    assert L.rbx_spaxel_sort_workspace_size(32 * 10 ** 6, 310_000) < 4 * (2 ** 24 + 3 * 32 * 10 ** 6) + 2 ** 20
     assert L.rbx_dot_workspace_size(10 ** 7) >= 8
 
 

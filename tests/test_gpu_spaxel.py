@@ -68,6 +68,27 @@ def test_sort_matches_stable_argsort(n, n_keys):
     np.testing.assert_array_equal(off.cpu().numpy(), eoff)
 
 
+@pytest.mark.parametrize("n,n_keys", [(1_000_000, 100_000), (16_000_000, 20_000_000)])
+def test_sort_radix_passes_match_stable_argsort(n, n_keys):
+    """Key counts whose (bucket, tile) count table would pass 2^24 entries
+    sort in LSD radix passes (2 passes of 16 + 1 bits for 1 M stars and
+    100 000 keys; 3 of 12 + 12 + 1 bits for 16 M stars and 2e7 keys), with
+    out-of-range ids (negative and >= n_keys) among them: the same stable
+    order, original ids and offsets as the one-pass sort."""
+    rng = np.random.default_rng(n % 1000 + 11)
+    ids = rng.integers(0, n_keys, n).astype(np.int32)
+    ids[: n // 4] = ids[n // 2]  # a long run of one key
+    bad = rng.integers(0, n, n // 50)
+    ids[bad[: len(bad) // 2]] = -1
+    ids[bad[len(bad) // 2:]] = n_keys + 3
+    perm, srt, off = ops.spaxel_sort(torch.as_tensor(ids, device=DEV), n_keys)
+    buckets = np.where((ids >= 0) & (ids < n_keys), ids, n_keys)
+    eperm, eoff = R.sort_by_spaxel(buckets, n_keys)
+    np.testing.assert_array_equal(perm.cpu().numpy(), eperm)
+    np.testing.assert_array_equal(srt.cpu().numpy(), ids[eperm])
+    np.testing.assert_array_equal(off.cpu().numpy(), eoff)
+
+
 def test_sort_out_of_range_ids_go_last_in_order():
     ids = np.array([3, 7, -1, 2, 9, 3, 0, 5], np.int32)  # n_keys = 5: 7, -1, 9, 5 overflow
     perm, srt, off = ops.spaxel_sort(torch.as_tensor(ids, device=DEV), 5)
