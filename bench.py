@@ -24,6 +24,7 @@ recomputes instead of storing can exceed.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -233,7 +234,7 @@ def summarize_counters(args):
             "copy_calibration": {"bytes": cal_bytes, "fetch_scale": round(fetch_scale, 5),
                                  "write_scale": round(write_scale, 5)}}
     out = [f"# Counters per launch: {label}", "",
-           "Commands (from /tmp with TMPDIR=/tmp, after `python -m rubix_amd.build`):", "", "```"]
+           "Commands (from the repository root with TMPDIR=/tmp, after `python -m rubix_amd.build`):", "", "```"]
     out += [c for c in args.command]
     out += ["```", "",
             f"Copy calibration: a 1 GiB device copy reads FETCH_SIZE x {fetch_scale:.4f} and "
@@ -283,11 +284,18 @@ def timed_leg(step, warmup, steps):
         step(None)
     torch.cuda.synchronize()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    t0 = time.perf_counter()
-    for i in range(steps):
-        step(evs[i])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    # no Python garbage collection inside the timed steps (a full collection
+    # of the autograd objects can stall the host for tens of ms)
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step(evs[i])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+    finally:
+        gc.enable()
     fwd = [a.elapsed_time(b) for a, b, _ in evs]
     adj = [b.elapsed_time(c) for _, b, c in evs]
     # start-to-start spans of the steps on the GPU (the last: start to end);
@@ -319,6 +327,9 @@ def main():
                     help="timed steps of the plain per-step GPU path (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
                     help="batched workload: default launch tuning, no autotuning")
+    ap.add_argument("--persist", type=int, default=None,
+                    help="batched workload: k persistent workgroups per CU for the per-star "
+                         "kernels (default tuning otherwise, no autotuning)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (twice): the reference "
                          "workload for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -719,14 +730,17 @@ def run_batched(args):
         cot = 2.0 * (cube.detach() - tgt) * wt
         torch.autograd.grad(cube, inputs, grad_outputs=cot)
 
+    fixed = ops.Tuning(persist=args.persist) if args.persist is not None else None
     if args.counters_run:
-        eager_step(ops.Tuning())
+        eager_step(fixed or ops.Tuning())
         torch.cuda.synchronize()
         return
     key = autotune.problem_key(L.n_total, L.n_rows, nt, n_src, dev,
                                n_templates=grid["flux"].shape[0] * grid["flux"].shape[1])
     t0 = time.perf_counter()
-    if args.no_tune:
+    if fixed is not None:
+        best, seen = fixed, {}
+    elif args.no_tune:
         best, seen = ops.Tuning(), {}
     else:
         best, seen = autotune.autotune(eager_step, key, reps=1, log=log)
@@ -742,11 +756,14 @@ def run_batched(args):
     for _ in range(args.warmup):
         step.replay()
     torch.cuda.synchronize()
+    gc.collect()
+    gc.disable()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step.replay()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    gc.enable()
     ms = elapsed / args.steps * 1e3
     work = L.n_total * nt
 
