@@ -1327,8 +1327,12 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
       uint64_t redo = 0;
       uint32_t jb = j0;
       if (interior) {
-        for (int gi = 0; gi < 64 && jb + 64 <= j1; ++gi, jb += 64)
-          if (!fast_group(jb)) redo |= uint64_t(1) << gi;
+        const uint32_t full = (j1 - j0) / 64u;
+        const uint32_t ng = full < 64u ? full : 64u;
+        for (uint32_t gi = 0; gi < ng; ++gi) {
+          if (!fast_group(j0 + 64u * gi)) redo |= uint64_t(1) << gi;  // rare, uniform
+        }
+        jb = j0 + 64u * ng;
       }
       while (redo) {
         const int gi = __builtin_ctzll(redo);
