@@ -309,7 +309,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=("single", "large", "batched", "noise", "dust"),
                     default="single",
                     help="single: 1M stars -> 50x50x3721 (the headline); large: 10M stars -> "
