@@ -768,7 +768,7 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
   __shared__ float W_s[kFusedWaves][kHalfWindow];  // shifted knots of each wavefront's window
   __shared__ float S_s[kFusedWaves][kHalfWindow];  // the star's spectrum at those knots
   __shared__ float4 st_s[4 * kStateBatch];
-  __shared__ float nn_s[2][4 * kFusedWaves];
+  __shared__ float nn_s[2][2][4 * kFusedWaves];  // [buffer][star of the pair][share]
   __shared__ float x_s[kFusedWaves][NB][64];   // the lanes' target wavelengths
   __shared__ float dt_s[kFusedWaves][NB][64];  // and bin widths
   static_assert(sizeof(StarState) == 4 * sizeof(float4), "a star state is four float4");
@@ -810,11 +810,14 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
       const float4* src = reinterpret_cast<const float4*>(state + b0);
       for (int i = threadIdx.x; i < 4 * nb_st; i += 64 * kFusedWaves) st_s[i] = src[i];
       __syncthreads();
-      for (int i = 0; i < nb_st; ++i) {
-        const int r = b0 + i;
+      // P_j of star i of the batch on this wavefront's bins; false (P = 0)
+      // for a dropped star (uniform: the same star for every wavefront)
+      auto star_values = [&](int i, float (&P)[NB], float& T, float& av) -> bool {
         const float4 s0 = st_s[4 * i], s1 = st_s[4 * i + 1], s2 = st_s[4 * i + 2];
         const int row = __builtin_amdgcn_readfirstlane(__float_as_int(s0.x));
-        if (row < 0) continue;  // the same star for every wavefront: uniform
+#pragma unroll
+        for (int t = 0; t < NB; ++t) P[t] = 0.f;
+        if (row < 0) return false;
         StarState s;
         s.row = row;
         s.a = uniform_f(s0.z);
@@ -824,11 +827,8 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
         s.w[2] = uniform_f(s1.z);
         s.w[3] = uniform_f(s1.w);
         s.m = uniform_f(s2.x);
-        s.T = uniform_f(s2.y);
-        s.av = kDust ? uniform_f(st_s[4 * i + 3].w) : 0.f;
-        float P[NB];
-#pragma unroll
-        for (int t = 0; t < NB; ++t) P[t] = 0.f;
+        T = uniform_f(s2.y);
+        av = kDust ? uniform_f(st_s[4 * i + 3].w) : 0.f;
         if (jh < jh_end) {
           const Rows R = rows_of(g, s.row);
           const Shift h = shift_of(g, s.a, s.inv_a);
@@ -884,19 +884,23 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
             }
           }
         }
+        return true;
+      };
+      // this wavefront's four row sums of Nn = sum_j P_j dtarget_j
+      auto post_share = [&](const float (&P)[NB], float* slot) {
         float nn = 0.f;
 #pragma unroll
         for (int t = 0; t < NB; ++t) nn += P[t] * dt_s[wave][t][lane];
-        // four shares per wavefront (its row sums), 64 in all; after the
-        // barrier every wavefront adds the 64 in the same fixed order
         nn = row_sum16(nn);
-        if ((lane & 15) == 0) nn_s[par][4 * wave + (lane >> 4)] = nn;
-        __syncthreads();  // every share of this star posted
-        const float Nn = wave_sum_dpp(nn_s[par][lane]);
-        par ^= 1;
+        if ((lane & 15) == 0) slot[4 * wave + (lane >> 4)] = nn;
+      };
+      // sf = nan_to_num(T / Nn) from the 64 shares (added in a fixed order),
+      // stored for the adjoint, and P_j sf into the f64 sums
+      auto accumulate = [&](const float (&P)[NB], float T, float av, const float* slot, int r) {
+        const float Nn = wave_sum_dpp(slot[lane]);
         // T / Nn with the hardware reciprocal (within one ulp of the division;
         // Nn = 0 still gives inf or nan for nan_to_num below)
-        float sf = s.T * __builtin_amdgcn_rcpf(Nn);
+        float sf = T * __builtin_amdgcn_rcpf(Nn);
         if (isnan(sf)) sf = 0.f;
         else if (isinf(sf)) sf = sf > 0.f ? FLT_MAX : -FLT_MAX;
         if (wave == 0 && lane == 0) {
@@ -906,9 +910,28 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
 #pragma unroll
         for (int t = 0; t < NB; ++t) {
           float v = P[t] * sf;
-          if (kDust && s.av != 0.f) v *= dust_factor(tau[t], s.av);
+          if (kDust && av != 0.f) v *= dust_factor(tau[t], av);
           acc[t] += double(v);
         }
+      };
+      // two stars per barrier: both stars' shares are posted, then one
+      // barrier; the stars are added in batch order as one at a time would
+      for (int i = 0; i < nb_st; i += 2) {
+        float PA[NB], PB[NB];
+        float TA = 0.f, TB = 0.f, avA = 0.f, avB = 0.f;
+        const bool va = star_values(i, PA, TA, avA);
+        const bool vb = i + 1 < nb_st && star_values(i + 1, PB, TB, avB);
+        if (!va && !vb) continue;
+        if (!vb) {
+#pragma unroll
+          for (int t = 0; t < NB; ++t) PB[t] = 0.f;
+        }
+        post_share(PA, nn_s[par][0]);
+        post_share(PB, nn_s[par][1]);
+        __syncthreads();  // every share of both stars posted
+        if (va) accumulate(PA, TA, avA, nn_s[par][0], b0 + i);
+        if (vb) accumulate(PB, TB, avB, nn_s[par][1], b0 + i + 1);
+        par ^= 1;
       }
     }
 #pragma unroll
