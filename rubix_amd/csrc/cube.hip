@@ -27,8 +27,9 @@
 //                 the chunk's stars accumulating P_j * sf in registers.
 //     cube_combine: cube[p, j] = sum of the spaxel's chunk partials in chunk
 //                 order (fixed order => bitwise deterministic).
-//   The per-(star, slab) kernels put lambda slab b % 8 on XCD b % 8, so each
-//   XCD's L2 holds only its slab of the templates.
+//   The per-(star, slab) kernels put lambda slab b % S on XCD b % 8 (S = 8
+//   for star_norm, 2 for the adjoint), so each XCD's L2 holds only its
+//   slab's share of the templates.
 //   Adjoint: star_bwd (wavefront / (star, lambda slab)) gathers
 //   dL/dcube[spaxel_i, :] over the slab's bins and accumulates seven sums of
 //   the combined template rows R_w (bilinear weights), R_u, R_s (their
@@ -51,7 +52,7 @@ namespace rbx {
 
 constexpr int kTile = 64;    // target bins per wavefront tile
 constexpr int kSlabs = 8;    // lambda slabs, one per XCD
-constexpr int kBwdSlabs = 4; // lambda slabs of the adjoint's (star, slab) items
+constexpr int kBwdSlabs = 2; // lambda slabs of the adjoint's (star, slab) items
 
 struct __align__(16) StarState {
   int32_t row;      // flux row of corner (iz-1, ja-1); -1 = contributes nothing
@@ -506,7 +507,7 @@ __device__ inline void slab_window(const SspView& g, const rbx_target& tg, uint3
 
 // Work items of the slab-mapped per-star kernels: item q is lambda slab
 // q % S of the W stars [(q / S) W, (q / S) W + W), one per wavefront of a
-// workgroup of W waves (S = 8 slabs for star_norm, 4 for the adjoint).
+// workgroup of W waves (S = 8 slabs for star_norm, 2 for the adjoint).
 // Workgroup b takes items b, b + G, b + 2G, ... with G = gridDim.x a
 // multiple of 8 (G = number of items, or a persistent grid of a few
 // workgroups per CU), so it always holds slab b % S and runs on XCD b % 8:
