@@ -76,6 +76,40 @@ def test_deterministic_bitwise(name):
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("chunk", [16, 65, 1024])
+def test_chunk_lengths_match_oracle(chunk):
+    """cfg1 in full (10k stars, up to 211 in one spaxel) with chunks of 16, 65
+    and 1024 stars: one-star remainders, odd star counts for the fused
+    kernel's pairs of stars, and chunks spanning several 64-star batches of
+    staged records.  Raw cube and the per-star gradients of <G, cube>
+    against the float64 oracle."""
+    w = cfg1(10_000)
+    t, ssp, tgt, edges = _device_inputs(w)
+    nb = w.geom["nb"]
+    ids = ops.spaxel_assign(t["coords"], edges)
+    layout = ops.BatchLayout([w.n_stars], [nb], DEV)
+    tune = ops.Tuning(chunk=chunk)
+    cube, ctx = ops.batch_forward(ssp, tgt, layout, t["mass"], t["age"], t["metallicity"],
+                                  t["velocity"], ids, tune=tune)
+    o = oracle("cfg1", "float64", 10_000)
+    e = rel_err(cube.cpu().numpy().reshape(o["cube"].shape), o["cube"])
+    print(f"cfg1 chunk {chunk}: raw cube rel64={e:.3e}")
+    assert e <= TOL
+    ref = ops.batch_forward(ssp, tgt, layout, t["mass"], t["age"], t["metallicity"],
+                            t["velocity"], ids)[0]
+    gen = torch.Generator(device=DEV).manual_seed(2)
+    G = torch.randn(cube.shape, generator=gen, device=DEV)
+    got = ops.batch_backward(ssp, tgt, layout, ctx, G, tune=tune)
+    _, ctx0 = ops.batch_forward(ssp, tgt, layout, t["mass"], t["age"], t["metallicity"],
+                                t["velocity"], ids)
+    want = ops.batch_backward(ssp, tgt, layout, ctx0, G)
+    # the chunking changes only the order of the cube's f64 partial sums; the
+    # per-star gradients do not depend on it
+    for a, b in zip(got, want):
+        assert torch.equal(a, b)
+    assert rel_err(cube.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+
+
 def test_edge_stars():
     """Out-of-grid stars (age 0 -> log10 = -inf, like filtered particles),
     zero mass, stars outside the field of view (clipped into the border
