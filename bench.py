@@ -173,6 +173,8 @@ LAUNCH_KERNELS = {
             "star_prep_kernel", "chunk_count_kernel", "chunk_fill_kernel", "cube_fused_kernel",
             "star_norm_kernel", "star_sf_kernel", "cube_accum_kernel", "cube_combine_kernel"),
     "bwd": ("star_bwd_reg_kernel", "star_bwd_kernel", "star_grad_kernel"),
+    "noise_fwd": ("flux_rows_kernel", "noise_s2n_kernel", "add_noise_kernel"),
+    "noise_bwd": ("noise_bwd_rows_kernel", "noise_coef_kernel", "add_rows_kernel"),
 }
 
 
@@ -873,6 +875,10 @@ def run_noise(args):
             ev[2].record()
         x.grad = None
 
+    if args.counters_run:
+        step(None)
+        torch.cuda.synchronize()
+        return
     _lib.TIMER.start()
     elapsed, fwd_ms, adj_ms = timed_leg(step, args.warmup, args.steps)
     calls = _lib.TIMER.stop()
@@ -884,12 +890,14 @@ def run_noise(args):
     fwd_b, adj_b = 12 * el, 20 * el
     dom = max(("rbx_apply_noise", fwd_b), ("rbx_apply_noise_bwd", adj_b),
               key=lambda kv: per_call.get(kv[0], 0.0))
-    k_ms = per_call.get(dom[0], float("nan"))
-    achieved = dom[1] / (k_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": dom[0], "kernel_ms": round(k_ms, 4),
-                "algorithmic_bytes_per_launch": dom[1]}
+    other = ("rbx_apply_noise", fwd_b) if dom[0] == "rbx_apply_noise_bwd" else \
+        ("rbx_apply_noise_bwd", adj_b)
+    # the draw (Threefry + erfinv per element) makes both calls VALU-bound
+    counters = load_counters("counters_noise_50x50x3721")
+    roofline = roofline_of(counters, dom[0], per_call.get(dom[0], float("nan")), dom[1])
+    if roofline is not None:
+        roofline["other_launch"] = roofline_of(counters, other[0],
+                                               per_call.get(other[0], float("nan")), other[1])
     cpu = None
     if not args.no_cpu:
         from oracle import ref_numpy as R
