@@ -42,6 +42,15 @@ def test_committed_counter_summaries_parse(name):
             assert k["fetch_kib"] >= 0 and k["write_kib"] >= 0
 
 
+def test_noise_counter_summary_parses():
+    c = bench.load_counters("counters_noise_50x50x3721")
+    assert c is not None
+    assert set(c["launches"]) == {"rbx_apply_noise", "rbx_apply_noise_bwd"}
+    assert set(c["launches"]["rbx_apply_noise"]["kernels"]) == set(bench.LAUNCH_KERNELS["noise_fwd"])
+    assert set(c["launches"]["rbx_apply_noise_bwd"]["kernels"]) == \
+        set(bench.LAUNCH_KERNELS["noise_bwd"])
+
+
 def test_roofline_arithmetic(monkeypatch):
     counters = {
         "source": "profiles/x.md",
