@@ -17,10 +17,10 @@
 //                 and T in O(1) from per-template prefix sums (cum).
 //     cube_fused  (workgroup of 16 wavefronts / spaxel chunk, two per lambda
 //                 slab): per star of the chunk, P_j on every bin once from the
-//                 knot window of each wavefront's 256 bins staged in its LDS,
-//                 Nn from the 16 shares after a barrier, sf = nan_to_num(T/Nn),
-//                 then P_j * sf into f64 registers.  No atomics.  (Ascending
-//                 targets of at most 4096 bins.)
+//                 knot window of each wavefront's 64 NB bins staged in its LDS,
+//                 Nn from 64 row shares after a barrier (one per two stars),
+//                 sf = nan_to_num(T/Nn), then P_j * sf into f64 registers.
+//                 No atomics.  (Ascending targets of at most 8192 bins.)
 //     otherwise   star_norm (wavefront / (star, lambda slab)): partial Nn of
 //                 the slab's bins, star_sf adds the 8 slabs in order; then
 //                 cube_accum (wavefront / (spaxel chunk, 64-bin tile)) walks
