@@ -1235,8 +1235,9 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         v[2] += gj * Rsv;
         v[3] += dj * Ruv;
         v[4] += dj * Rsv;
-        v[5] += (-gj * kap) * dw;
-        v[6] += (-dj * kap) * dw;
+        const float kd = kap * dw;  // the Doppler change of R_w across the segment
+        v[5] -= gj * kd;
+        v[6] -= dj * kd;
         return true;
       };
       // the general path of one group (any slab, full or the last partial one)
@@ -1319,8 +1320,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
             const float w0 = lane_of(rv.w, li), w1 = lane_of(rv.w, li + 1);
             const float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
             const float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
-            const float gk = -gj * kap, dk = -dj * kap;
             const float dw = w1 - w0;
+            const float kd = kap * dw;  // the Doppler change of R_w across the segment
             const float Rwv = w0 + th * dw;
             const float Ruv = u0 + th * (u1 - u0);
             const float Rsv = s0 + th * (s1 - s0);
@@ -1329,8 +1330,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
             v[2] += gj * Rsv;
             v[3] += dj * Ruv;
             v[4] += dj * Rsv;
-            v[5] += gk * dw;
-            v[6] += dk * dw;
+            v[5] -= gj * kd;
+            v[6] -= dj * kd;
           }
         }
         if (gather && valid) bwd_bin_gather(g, R, s, B, h, x, gj, dj, v);
