@@ -1126,6 +1126,25 @@ __device__ inline float lane_of(float v, int l) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(l << 2, __float_as_int(v)));
 }
 
+// The next lane's v (DPP wave shift left by one); lane 63 reads 0.
+__device__ inline float next_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+
+// Change of the combined row R_w from this lane's knot to the next lane's,
+// sum_c w_c (f_c[k + 1] - f_c[k]) from the corners' own knot-to-knot
+// differences: exact in float32 for neighbouring values within a factor 2,
+// unlike the difference of the two rounded rows, which loses the digits
+// the Doppler sums need where the rows change little from knot to knot.
+// Lanes whose next lane does not hold the next knot (the window's last, and
+// lanes past it) give a value that is never used.
+__device__ inline float w_change(const float (&f)[4], const BwdRows& B) {
+  float d[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) d[c] = next_lane(f[c]) - f[c];
+  return combine_rows(d[0], d[1], d[2], d[3], B.fu, B.fs).w;
+}
+
 // star_bwd_kernel for ascending targets with a register window: for each
 // group of 64 consecutive bins (one per lane) the knots the group can touch,
 // [k0(first bin) - 1, k0(last bin) + 2] from the compact locator, are
@@ -1197,8 +1216,15 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         const int kl = wb + t.lane;
         const uint32_t kk = uint32_t(kl < we ? kl : we);
         const float lamk = ld_at(g.wave, kk);
-        const RowVals rv = combine_rows(ld_at(R.f[0], kk), ld_at(R.f[1], kk), ld_at(R.f[2], kk),
-                                        ld_at(R.f[3], kk), B.fu, B.fs);
+        float fc[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) fc[c] = ld_at(R.f[c], kk);
+        const RowVals rv = combine_rows(fc[0], fc[1], fc[2], fc[3], B.fu, B.fs);
+        // the change of R_w to the next knot (the next lane's) from the
+        // corners' own knot-to-knot differences (exact in float32 for
+        // neighbouring values), not from the rounded rows: the Doppler sums
+        // cancel strongly across bins
+        const float dwk = w_change(fc, B);
         const float y = x * h.inv_a;
         int li = k0 - wb;  // in [1, span - 3]: the guesses are monotone in the bin
         li += lane_of(lamk, li + 1) <= y ? 1 : 0;
@@ -1224,10 +1250,9 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         const float r = __builtin_amdgcn_rcpf(wr - wl);
         const float th = (x - wl) * r;
         const float kap = (l0 + th * (l1 - l0)) * r;
-        const float w0 = lane_of(rv.w, li), w1 = lane_of(rv.w, li + 1);
+        const float w0 = lane_of(rv.w, li), dw = lane_of(dwk, li);
         const float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
         const float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
-        const float dw = w1 - w0;
         const float Ruv = u0 + th * (u1 - u0);
         const float Rsv = s0 + th * (s1 - s0);
         v[0] += gj * (w0 + th * dw);
@@ -1267,8 +1292,13 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
           const int kl = wb + t.lane;
           const uint32_t kk = uint32_t(kl < we ? kl : we);
           const float lamk = ld_at(g.wave, kk);
-          const RowVals rv = combine_rows(ld_at(R.f[0], kk), ld_at(R.f[1], kk),
-                                          ld_at(R.f[2], kk), ld_at(R.f[3], kk), B.fu, B.fs);
+          // the knot and the next one (the last knot of the grid pairs with
+          // the one before it; its change to a next knot is never used)
+          float fc[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) fc[c] = ld_at(R.f[c], kk);
+          const RowVals rv = combine_rows(fc[0], fc[1], fc[2], fc[3], B.fu, B.fs);
+          const float dwk = w_change(fc, B);  // exact change to the next knot, as above
           // exact segment on the window's knots (as win_knots)
           const float y = x * h.inv_a;
           int li = k0 - wb;
@@ -1317,10 +1347,9 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
             const bool right = !kIn && (i == n_w - 1) && (x > wr);
             const float th = left ? 0.f : (right ? 1.f : th0);
             const float kap = (left || right || flat) ? 0.f : (l0 + th0 * (l1 - l0)) * r;
-            const float w0 = lane_of(rv.w, li), w1 = lane_of(rv.w, li + 1);
+            const float w0 = lane_of(rv.w, li), dw = lane_of(dwk, li);
             const float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
             const float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
-            const float dw = w1 - w0;
             const float kd = kap * dw;  // the Doppler change of R_w across the segment
             const float Rwv = w0 + th * dw;
             const float Ruv = u0 + th * (u1 - u0);

@@ -128,6 +128,50 @@ def test_long_axis_two_kernel_path(template):
             assert e <= TOL, (label, nm, e)
 
 
+@pytest.mark.parametrize("n_bins,nb_lane", [(4500, 5), (5800, 6), (8000, 8)])
+def test_fused_forward_bins_per_lane(n_bins, nb_lane):
+    """The fused forward's wider shapes (NB = 5, 6, 8 bins per lane; MaNGA's
+    6474 bins take NB = 7 above): ascending axes of 4500, 5800 and 8000 bins
+    over MaNGA_32's range with the 12 x 100 x 5000 grid.  Raw cube against
+    the float64 oracle and the per-star gradients of <G, cube> against the
+    float64 gradient oracle, <= 1e-5."""
+    tiles_per_slab = -(-(-(-n_bins // 64)) // 8)  # 64-bin tiles in each of 8 slabs
+    assert nb_lane == max(4, (tiles_per_slab + 1) // 2)  # the host's choice of NB
+    w = S.instrument_workload("MaNGA_32", "synthetic_12x100x5000", 1200, psf_sigma=2.0,
+                              psf_size=11)
+    w.geom["wave_target"] = np.linspace(3622.0, 10354.0, n_bins).astype(np.float32)
+    g = w.grid
+    nb, nt = w.geom["nb"], n_bins
+    ssp = ops.PreparedSSP(g["flux"], g["ax_met"], g["ax_age"], g["wave"], w.geom["z"],
+                          g["log_axes"], DEV)
+    tg = ops.TargetGrid(w.geom["wave_target"], DEV)
+    t = {k: torch.as_tensor(v, device=DEV) for k, v in w.stars.items()}
+    ids = ops.spaxel_assign(t["coords"], torch.as_tensor(w.geom["edges"], device=DEV))
+    layout = ops.BatchLayout([w.n_stars], [nb], DEV)
+    cube, ctx = ops.batch_forward(ssp, tg, layout, t["mass"], t["age"], t["metallicity"],
+                                  t["velocity"], ids)
+    ref = R.plain_forward(w.stars, g, w.geom, dtype=np.float64, raw_only=True)
+    e = rel_err(cube.cpu().numpy().reshape(nb, nb, nt), ref)
+    print(f"{n_bins} bins (NB {nb_lane}): raw cube rel64={e:.3e}")
+    assert e <= TOL
+    gen = torch.Generator(device=DEV).manual_seed(11)
+    Gc = torch.randn(nb * nb, nt, generator=gen, device=DEV)
+    st = w.stars
+    f64 = torch.float64
+    T = {k: torch.tensor(st[k], dtype=f64, requires_grad=True) for k in ("mass", "age", "metallicity")}
+    vl = torch.tensor(st["velocity"][:, 2], dtype=f64, requires_grad=True)
+    raw = RT.star_cube(T["mass"], T["age"], T["metallicity"], vl,
+                       torch.tensor(R.square_spaxel_assignment(st["coords"], w.geom["edges"])
+                                    .astype(np.int64)), g, w.geom, f64)
+    raw.backward(torch.tensor(Gc.cpu().numpy().reshape(nb, nb, nt), dtype=f64))
+    grads = ops.batch_backward(ssp, tg, layout, ctx, Gc)
+    for nm, a, b in zip(("mass", "age", "metallicity", "v_los"), grads,
+                        (T["mass"].grad, T["age"].grad, T["metallicity"].grad, vl.grad)):
+        e = rel_err(a.cpu().numpy(), b.numpy())
+        print(f"{n_bins} bins: d/d{nm} rel={e:.3e}")
+        assert e <= TOL, (nm, e)
+
+
 @pytest.mark.parametrize("name", sorted(telescopes_config))
 def test_calc_ifu_fused_runs_for_every_instrument(name):
     tel = telescopes_config[name]
