@@ -209,6 +209,15 @@ __device__ inline Pair ld_pair(const float* base, uint32_t idx) {
   return *reinterpret_cast<const Pair*>(reinterpret_cast<const char*>(base) + idx * 4u);
 }
 
+// Elements idx .. idx + 3 in one 16-byte load (4-byte alignment, as above).
+struct alignas(4) Quad {
+  float v[4];
+};
+
+__device__ inline Quad ld_quad(const float* base, uint32_t idx) {
+  return *reinterpret_cast<const Quad*>(reinterpret_cast<const char*>(base) + idx * 4u);
+}
+
 // Per-star constants of the search: a, 1/a and the locator cell of target x
 // as x * c1 + c0.
 struct Shift {
@@ -430,12 +439,22 @@ __device__ inline void win_values(const float (&x)[NB], const int (&k0)[NB], int
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
     int l = k0[t] - k_lo;
-    if (!kInterior) l = l < 0 ? 0 : (l > li_max ? li_max : l);
-    l += W[l + 1] <= x[t] ? 1 : 0;
-    if (!kInterior) l = l > li_max ? li_max : l;
-    li[t] = l;
-    wl[t] = W[l];
-    wr[t] = W[l + 1];
+    if (kInterior) {
+      // the three knots of both candidate segments in one round of reads,
+      // then the next-knot test picks one (no read waits on the test)
+      const float w0 = W[l], w1 = W[l + 1], w2 = W[l + 2];
+      const bool step = w1 <= x[t];
+      li[t] = l + (step ? 1 : 0);
+      wl[t] = step ? w1 : w0;
+      wr[t] = step ? w2 : w1;
+    } else {
+      l = l < 0 ? 0 : (l > li_max ? li_max : l);
+      l += W[l + 1] <= x[t] ? 1 : 0;
+      l = l > li_max ? li_max : l;
+      li[t] = l;
+      wl[t] = W[l];
+      wr[t] = W[l + 1];
+    }
     // interior: l lies in [3, nk - 5] (the window's margins), inside both
     // bounds of the step, so a step is needed exactly when x is off [w_l, w_r)
     if (kInterior)
@@ -848,22 +867,41 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
             slab_window(g, tg, jh, jh_end, h, &k_lo, &nk);
           }
           if (nk <= kHalfWindow) {
-            for (int k = lane; k < nk; k += 64) {
-              const uint32_t kk = uint32_t(k_lo + k);
-              const float lw = ld_at(g.wave, kk);
-              float fc[4];
+            // shifted knot as the reference's wave * a; S_k = F_k m,
+            // rounded once per knot as the reference's spectra
+            if (nk >= 4) {
+              // four knots per lane in 16-byte loads: windows up to 256
+              // knots in one round of loads.  The last group ends on the
+              // window's last knot (b = nk - 4), so no load leaves the grid;
+              // its knots overlap the lane before, which stores the same values.
+              for (int k = 4 * lane; k < nk; k += 256) {
+                const int b = k + 4 <= nk ? k : nk - 4;
+                const uint32_t kk = uint32_t(k_lo + b);
+                const Quad lw = ld_quad(g.wave, kk);
+                Quad fc[4];
 #pragma unroll
-              for (int c = 0; c < 4; ++c) fc[c] = ld_at(R.f[c], kk);
-              // the five loads in flight together, then the arithmetic
-              __builtin_amdgcn_sched_group_barrier(0x020, 5, 0);
-              __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
-              float f = 0.f;
+                for (int c = 0; c < 4; ++c) fc[c] = ld_quad(R.f[c], kk);
+                // the five loads in flight together, then the arithmetic
+                __builtin_amdgcn_sched_group_barrier(0x020, 5, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 32, 0);
 #pragma unroll
-              for (int c = 0; c < 4; ++c) f += s.w[c] * fc[c];
-              // shifted knot as the reference's wave * a; S_k = F_k m,
-              // rounded once per knot as the reference's spectra
-              W[k] = mul_nocontract(lw, s.a);
-              S[k] = f * s.m;
+                for (int e = 0; e < 4; ++e) {
+                  float f = 0.f;
+#pragma unroll
+                  for (int c = 0; c < 4; ++c) f += s.w[c] * fc[c].v[e];
+                  W[b + e] = mul_nocontract(lw.v[e], s.a);
+                  S[b + e] = f * s.m;
+                }
+              }
+            } else {
+              for (int k = lane; k < nk; k += 64) {
+                const uint32_t kk = uint32_t(k_lo + k);
+                float f = 0.f;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
+                W[k] = mul_nocontract(ld_at(g.wave, kk), s.a);
+                S[k] = f * s.m;
+              }
             }
             wave_lds_sync();
             float x[NB];
