@@ -1174,13 +1174,15 @@ __device__ inline float next_lane(float v) {
 // differences: exact in float32 for neighbouring values within a factor 2,
 // unlike the difference of the two rounded rows, which loses the digits
 // the Doppler sums need where the rows change little from knot to knot.
+// The exact differences are weighted with the star's bilinear corner weights
+// wc (four products instead of the difference form's seven operations).
 // Lanes whose next lane does not hold the next knot (the window's last, and
 // lanes past it) give a value that is never used.
-__device__ inline float w_change(const float (&f)[4], const BwdRows& B) {
-  float d[4];
+__device__ inline float w_change(const float (&f)[4], const float (&wc)[4]) {
+  float r = 0.f;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) d[c] = next_lane(f[c]) - f[c];
-  return combine_rows(d[0], d[1], d[2], d[3], B.fu, B.fs).w;
+  for (int c = 0; c < 4; ++c) r += wc[c] * (next_lane(f[c]) - f[c]);
+  return r;
 }
 
 // star_bwd_kernel for ascending targets with a register window: for each
@@ -1226,6 +1228,9 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
       BwdRows B;
       B.fu = gradw[t.r * 2];
       B.fs = gradw[t.r * 2 + 1];
+      float wc[4];  // the corner weights, wave-uniform
+#pragma unroll
+      for (int c = 0; c < 4; ++c) wc[c] = uniform_f(s.w[c]);
       const uint32_t j0 = uint32_t(t.slab * slab_bins);
       const uint32_t j1 = j0 + slab_bins < uint32_t(tg.n) ? j0 + slab_bins : uint32_t(tg.n);
       // interior slab: every bin's segment has knots in [1, n - 2] (x ascending
@@ -1237,16 +1242,14 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         const float wn = mul_nocontract(ld_at(g.wave, uint32_t(n_w - 2)), h.a);
         interior = n_w >= 4 && ld_at(tg.wave, j0) >= w1 && ld_at(tg.wave, j1 - 1) < wn;
       }
-      // the short path of a full interior group; false (nothing added) when
-      // the group needs the general path
-      auto fast_group = [&](uint32_t jb) -> bool {
+      // the short path of a full interior group, given its bins' (x, dtarget)
+      // pair, dL/dcube value and locator guess; false (nothing added) when
+      // the group needs the general path.  mid() runs once the group's knots
+      // are in (the next group's locator guess goes there).
+      auto fast_group = [&](uint32_t jb, Pair xd, float gj, int k0, auto&& mid) -> bool {
         const uint32_t j = jb + uint32_t(t.lane);
-        const Pair xd = ld_pair(tg.pairs, 2 * j);
         const float x = xd.lo, dj = xd.hi;
-        float gj = ld_at(grow, j);
         if (kDust && s.av != 0.f) gj *= dust_factor(dust_tau(axav, j), s.av);
-        const float fm = __builtin_amdgcn_fmed3f(fmaf(x, h.c1, h.c0), 0.f, float(g.n_loc - 1));
-        const int k0 = ld_at(g.loc + 4 * g.n_loc, uint32_t(fm));
         const int wb = __builtin_amdgcn_readfirstlane(k0) - 1;
         const int we = __builtin_amdgcn_readlane(k0, 63) + 2;
         const int span = we - wb + 1;
@@ -1262,7 +1265,8 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         // corners' own knot-to-knot differences (exact in float32 for
         // neighbouring values), not from the rounded rows: the Doppler sums
         // cancel strongly across bins
-        const float dwk = w_change(fc, B);
+        const float dwk = w_change(fc, wc);
+        mid();
         const float y = x * h.inv_a;
         int li = k0 - wb;  // in [1, span - 3]: the guesses are monotone in the bin
         li += lane_of(lamk, li + 1) <= y ? 1 : 0;
@@ -1336,7 +1340,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
 #pragma unroll
           for (int c = 0; c < 4; ++c) fc[c] = ld_at(R.f[c], kk);
           const RowVals rv = combine_rows(fc[0], fc[1], fc[2], fc[3], B.fu, B.fs);
-          const float dwk = w_change(fc, B);  // exact change to the next knot, as above
+          const float dwk = w_change(fc, wc);  // exact change to the next knot, as above
           // exact segment on the window's knots (as win_knots)
           const float y = x * h.inv_a;
           int li = k0 - wb;
@@ -1421,8 +1425,42 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
       if (interior) {
         const uint32_t full = (j1 - j0) / 64u;
         const uint32_t ng = full < 64u ? full : 64u;
-        for (uint32_t gi = 0; gi < ng; ++gi) {
-          if (!fast_group(j0 + 64u * gi)) redo |= uint64_t(1) << gi;  // rare, uniform
+        // the loads that do not depend on the group before run ahead: the
+        // next group's (x, dtarget) pairs and dL/dcube values are issued with
+        // this group's knot loads, its locator guesses (which need x) once
+        // these knots are in, so a group waits on one round of loads instead
+        // of three in a row.  The next group's index is clamped to the last
+        // full group (in bounds; the extra loads are not used).  Two groups
+        // per pass, the inputs alternating between the A and B registers
+        // (no copies between groups).
+        const uint32_t jl = j0 + uint32_t(t.lane);
+        auto guess = [&](float x) {
+          const float fm = __builtin_amdgcn_fmed3f(fmaf(x, h.c1, h.c0), 0.f, float(g.n_loc - 1));
+          return int(ld_at(g.loc + 4 * g.n_loc, uint32_t(fm)));
+        };
+        auto run = [&](uint32_t gi, const Pair& xd, float gj, int k0, Pair& xdn, float& gjn,
+                       int& k0n) {
+          const uint32_t bn = jl + 64u * (gi + 1 < ng ? gi + 1 : ng - 1);
+          xdn = ld_pair(tg.pairs, 2 * bn);
+          gjn = ld_at(grow, bn);
+          bool guessed = false;
+          auto mid = [&] {
+            k0n = guess(xdn.lo);
+            guessed = true;
+          };
+          if (!fast_group(j0 + 64u * gi, xd, gj, k0, mid)) redo |= uint64_t(1) << gi;  // rare, uniform
+          if (!guessed) k0n = guess(xdn.lo);
+        };
+        if (ng > 0) {
+          Pair xdA = ld_pair(tg.pairs, 2 * jl), xdB;
+          float gjA = ld_at(grow, jl), gjB;
+          int k0A = guess(xdA.lo), k0B;
+          uint32_t gi = 0;
+          for (; gi + 1 < ng; gi += 2) {
+            run(gi, xdA, gjA, k0A, xdB, gjB, k0B);
+            run(gi + 1, xdB, gjB, k0B, xdA, gjA, k0A);
+          }
+          if (gi < ng) run(gi, xdA, gjA, k0A, xdB, gjB, k0B);
         }
         jb = j0 + 64u * ng;
       }
