@@ -9,12 +9,15 @@ Inputs are resident in HBM before timing starts.
 
 Prints one JSON line.  Metric: star*lambda-bins/s = N * n_target / t(fwd+adj).
 
-Roofline of the dominant launch: the per-star kernels issue one VALU wave-
-instruction per 4 cycles per SIMD at best and are bound by that issue rate
-(their HBM traffic is a few per cent of the 8 TB/s peak), so `bound` is
-"valu": achieved = VALU wave-instructions of the launch (rocprofv3
-SQ_INSTS_VALU x SQ_WAVES, committed under profiles/) / the launch's duration
-measured here with HIP events; peak = 4 SIMDs x CUs x clock / 4 cycles.
+Roofline of the dominant launch: the per-star kernels are bound by the VALU
+issue slots (their HBM traffic is a few per cent of the 8 TB/s peak), so
+`bound` is "valu".  A SIMD issues a wave64 VALU instruction every 4 cycles, or
+two in one 4-cycle slot when the hardware pairs them (rocprofv3
+SQ_ACTIVE_INST_VALU2 counts those slots).  achieved = the launch's VALU issue
+slots (SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2, totals over its waves, committed
+under profiles/) / the launch's duration measured here with HIP events;
+peak = 4 SIMDs x CUs x clock / 4 cycles, so frac <= 1 and frac = 1 is every
+SIMD issuing in every slot.  `valu_wave_instructions` is the instruction count.
 `hbm_frac_physical` is the launch's measured HBM bytes (FETCH_SIZE and
 WRITE_SIZE, calibrated against a device copy) over its duration and 8 TB/s;
 `effective_frac` the plain per-step byte model (every intermediate written
@@ -40,7 +43,7 @@ METRIC = "star·λ-bins/s fwd+adjoint, 1M stars→50²×3721 cube; % HBM rooflin
 HBM_PEAK_GBS = 8000.0
 CLOCK_GHZ = 2.4          # MI355X peak engine clock
 SIMDS_PER_CU = 4
-CYCLES_PER_VALU = 4      # a wave64 VALU instruction on a 16-lane SIMD
+CYCLES_PER_VALU = 4      # one VALU issue slot: a wave64 instruction on a 16-lane SIMD
 
 
 def log(*a):
@@ -151,15 +154,19 @@ def roofline_of(counters, call, call_ms, algo_bytes):
     L = counters["launches"][call]
     cal = counters["copy_calibration"]
     valu = sum(k["waves"] * k["valu_per_wave"] for k in L["kernels"].values())
+    # 4-cycle slots that issued two VALU instructions (0 where not counted)
+    dual = sum(k["waves"] * k.get("valu2_per_wave", 0.0) for k in L["kernels"].values())
+    slots = valu - dual
     traffic = sum(k["fetch_kib"] * 1024 * cal["fetch_scale"] + k["write_kib"] * 1024 * cal["write_scale"]
                   for k in L["kernels"].values())
     peak, cus = valu_peak_per_s()
     t = call_ms * 1e-3
-    achieved = valu / t
+    achieved = slots / t
     return {"bound": "valu", "achieved": round(achieved / 1e9, 2), "peak": round(peak / 1e9, 2),
-            "unit": "G wave-instructions/s", "frac": round(achieved / peak, 4),
+            "unit": "G VALU issue slots/s", "frac": round(achieved / peak, 4),
             "traffic": round(traffic), "kernel": call, "kernels": sorted(L["kernels"]),
             "kernel_ms": round(call_ms, 4), "valu_wave_instructions": round(valu),
+            "valu_dual_issue_slots": round(dual), "valu_issue_slots": round(slots),
             "hbm_frac_physical": round(traffic / t / (HBM_PEAK_GBS * 1e9), 4),
             "effective_frac": round(algo_bytes / t / (HBM_PEAK_GBS * 1e9), 4),
             "algorithmic_bytes_per_launch": algo_bytes, "cus": cus, "clock_ghz": CLOCK_GHZ,
@@ -197,7 +204,9 @@ def summarize_counters(args):
     """Markdown summary (with a JSON block bench.py reads) of rocprofv3 PMC
     runs of one workload: per C-ABI launch (--launch NAME=COUNT:fwd|bwd, COUNT
     launches of it in the run) the total SQ_WAVES, SQ_INSTS_VALU, FETCH_SIZE and
-    WRITE_SIZE of its kernels per launch, and the copy calibration."""
+    WRITE_SIZE of its kernels per launch (and SQ_ACTIVE_INST_VALU2,
+    SQ_INSTS_VALU_INT32 where the VALU pass counted them), and the copy
+    calibration."""
     per = {}
     for kind, d in (("valu", args.valu_dir), ("fetch", args.fetch_dir), ("write", args.write_dir)):
         for r in _pmc_rows(d):
@@ -230,6 +239,10 @@ def summarize_counters(args):
                          "fetch_kib": e.get("FETCH_SIZE", 0.0) / count,
                          "write_kib": e.get("WRITE_SIZE", 0.0) / count,
                          "dispatches_per_launch": e["dispatches"] / count}
+                if "SQ_ACTIVE_INST_VALU2" in e:
+                    ks[k]["valu2_per_wave"] = e["SQ_ACTIVE_INST_VALU2"] / e["SQ_WAVES"]
+                if "SQ_INSTS_VALU_INT32" in e:
+                    ks[k]["int32_per_wave"] = e["SQ_INSTS_VALU_INT32"] / e["SQ_WAVES"]
         launches[name] = {"kernels": ks}
     label = args.workload_label or args.workload
     data = {"workload": label, "launches": launches,
@@ -242,12 +255,17 @@ def summarize_counters(args):
             f"Copy calibration: a 1 GiB device copy reads FETCH_SIZE x {fetch_scale:.4f} and "
             f"WRITE_SIZE x {write_scale:.4f} = 1 GiB per dispatch; the bytes below are scaled by "
             "these factors.", "",
-            "| launch | kernel | waves | VALU / wave | VALU wave-instr | HBM read B | HBM written B |",
-            "|---|---|---|---|---|---|---|"]
+            "VALU issue slots = VALU instructions - dual-issue slots (SQ_ACTIVE_INST_VALU2: "
+            "4-cycle slots in which a SIMD issued two VALU instructions).", "",
+            "| launch | kernel | waves | VALU / wave | dual-issue slots / wave | VALU issue slots | "
+            "HBM read B | HBM written B |",
+            "|---|---|---|---|---|---|---|---|"]
     for ln, L in launches.items():
         for k, e in sorted(L["kernels"].items(), key=lambda kv: -kv[1]["waves"] * kv[1]["valu_per_wave"]):
-            out.append(f"| {ln} | {k} | {e['waves']:.4g} | {e['valu_per_wave']:.1f} | "
-                       f"{e['waves'] * e['valu_per_wave']:.4g} | {e['fetch_kib'] * 1024 * fetch_scale:.4g} | "
+            v2 = e.get("valu2_per_wave", 0.0)
+            out.append(f"| {ln} | {k} | {e['waves']:.4g} | {e['valu_per_wave']:.1f} | {v2:.1f} | "
+                       f"{e['waves'] * (e['valu_per_wave'] - v2):.4g} | "
+                       f"{e['fetch_kib'] * 1024 * fetch_scale:.4g} | "
                        f"{e['write_kib'] * 1024 * write_scale:.4g} |")
     out += ["", "```json", json.dumps(data, indent=1), "```"]
     print("\n".join(out))

@@ -869,11 +869,12 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
           if (nk <= kHalfWindow) {
             // shifted knot as the reference's wave * a; S_k = F_k m,
             // rounded once per knot as the reference's spectra
-            if (nk >= 4) {
-              // four knots per lane in 16-byte loads: windows up to 256
-              // knots in one round of loads.  The last group ends on the
-              // window's last knot (b = nk - 4), so no load leaves the grid;
-              // its knots overlap the lane before, which stores the same values.
+            if (nk > 128) {
+              // wide windows (knots denser than the bins): four knots per
+              // lane in 16-byte loads, one round of loads up to 256 knots.
+              // The last group ends on the window's last knot (b = nk - 4),
+              // so no load leaves the grid; its knots overlap the lane
+              // before, which stores the same values.
               for (int k = 4 * lane; k < nk; k += 256) {
                 const int b = k + 4 <= nk ? k : nk - 4;
                 const uint32_t kk = uint32_t(k_lo + b);
@@ -894,12 +895,20 @@ cube_fused_kernel(SspView g, rbx_target tg, StarState* __restrict__ state,
                 }
               }
             } else {
+              // narrow windows: one knot per lane (a quarter of the
+              // arithmetic per round when the window fits 64 or 128 lanes)
               for (int k = lane; k < nk; k += 64) {
                 const uint32_t kk = uint32_t(k_lo + k);
+                const float lw = ld_at(g.wave, kk);
+                float fc[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) fc[c] = ld_at(R.f[c], kk);
+                __builtin_amdgcn_sched_group_barrier(0x020, 5, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 8, 0);
                 float f = 0.f;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) f += s.w[c] * ld_at(R.f[c], kk);
-                W[k] = mul_nocontract(ld_at(g.wave, kk), s.a);
+                for (int c = 0; c < 4; ++c) f += s.w[c] * fc[c];
+                W[k] = mul_nocontract(lw, s.a);
                 S[k] = f * s.m;
               }
             }

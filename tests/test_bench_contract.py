@@ -40,6 +40,8 @@ def test_committed_counter_summaries_parse(name):
         for k in L["kernels"].values():
             assert k["waves"] > 0 and k["valu_per_wave"] > 0
             assert k["fetch_kib"] >= 0 and k["write_kib"] >= 0
+            # a dual-issue slot holds two of the kernel's instructions
+            assert 0 <= k["valu2_per_wave"] <= k["valu_per_wave"] / 2
 
 
 def test_noise_counter_summary_parses():
@@ -56,18 +58,23 @@ def test_roofline_arithmetic(monkeypatch):
         "source": "profiles/x.md",
         "copy_calibration": {"fetch_scale": 2.0, "write_scale": 1.0},
         "launches": {"rbx_cube_bwd": {"kernels": {
-            "a": {"waves": 1000.0, "valu_per_wave": 700.0, "fetch_kib": 1024.0, "write_kib": 512.0},
+            "a": {"waves": 1000.0, "valu_per_wave": 700.0, "valu2_per_wave": 150.0,
+                  "fetch_kib": 1024.0, "write_kib": 512.0},
             "b": {"waves": 10.0, "valu_per_wave": 100.0, "fetch_kib": 0.0, "write_kib": 1024.0},
         }}},
     }
     monkeypatch.setattr(bench, "valu_peak_per_s", lambda: (1.0e9, 256))
     r = bench.roofline_of(counters, "rbx_cube_bwd", 2.0, 8.0e9)
     valu = 1000 * 700 + 10 * 100
+    # a slot that issued two instructions counts once; kernel b has no dual-issue count
+    slots = valu - 1000 * 150
     assert r["bound"] == "valu"
     assert r["valu_wave_instructions"] == valu
-    # achieved in G wave-instructions/s (2 decimals), frac to 4 decimals
-    assert r["achieved"] == pytest.approx(valu / 2e-3 / 1e9, abs=0.006)
-    assert r["frac"] == pytest.approx(valu / 2e-3 / 1e9, abs=1e-4)
+    assert r["valu_dual_issue_slots"] == 1000 * 150
+    assert r["valu_issue_slots"] == slots
+    # achieved in G issue slots/s (2 decimals), frac to 4 decimals
+    assert r["achieved"] == pytest.approx(slots / 2e-3 / 1e9, abs=0.006)
+    assert r["frac"] == pytest.approx(slots / 2e-3 / 1e9, abs=1e-4)
     traffic = (1024 * 2.0 + 512 + 1024) * 1024
     assert r["traffic"] == round(traffic)
     assert r["hbm_frac_physical"] == pytest.approx(traffic / 2e-3 / 8e12, abs=1e-4)
@@ -98,6 +105,10 @@ def test_summarize_counters_on_synthetic_runs(tmp_path, capsys):
         {"Dispatch_Id": 2, "Kernel_Name": k_fwd, "Counter_Name": "SQ_INSTS_VALU", "Counter_Value": 5000},
         {"Dispatch_Id": 3, "Kernel_Name": k_bwd, "Counter_Name": "SQ_WAVES", "Counter_Value": 40},
         {"Dispatch_Id": 3, "Kernel_Name": k_bwd, "Counter_Name": "SQ_INSTS_VALU", "Counter_Value": 8000},
+        {"Dispatch_Id": 3, "Kernel_Name": k_bwd, "Counter_Name": "SQ_ACTIVE_INST_VALU2",
+         "Counter_Value": 2000},
+        {"Dispatch_Id": 3, "Kernel_Name": k_bwd, "Counter_Name": "SQ_INSTS_VALU_INT32",
+         "Counter_Value": 800},
     ])
     _pmc_csv(d["fetch"], [
         {"Dispatch_Id": 1, "Kernel_Name": k_fwd, "Counter_Name": "FETCH_SIZE", "Counter_Value": 10},
@@ -127,6 +138,10 @@ def test_summarize_counters_on_synthetic_runs(tmp_path, capsys):
     assert f["fetch_kib"] == 20 and f["write_kib"] == 2 and f["dispatches_per_launch"] == 1
     b = data["launches"]["rbx_cube_bwd"]["kernels"]["star_bwd_reg_kernel<4, false>"]
     assert b["waves"] == 40 and b["valu_per_wave"] == 200 and b["fetch_kib"] == 7
+    assert b["valu2_per_wave"] == 50 and b["int32_per_wave"] == 20
+    assert "valu2_per_wave" not in f
+    # the table's issue-slot column: waves x (VALU - dual-issue slots) per wave
+    assert "| rbx_cube_bwd | star_bwd_reg_kernel<4, false> | 40 | 200.0 | 50.0 | 6000 |" in out
 
 
 def test_more_than_one_gpu_is_refused():
