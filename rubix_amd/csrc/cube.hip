@@ -719,13 +719,14 @@ constexpr int kMaxFusedBins = 8;
 // is spaxel chunk q, one workgroup of 16 wavefronts, two per lambda slab
 // (wavefront 2b + h takes the bins j0 + 64 NB h + lane + 64 t, t < NB, of
 // slab b; NB = 4 up to 4096 target bins).  The chunk's star states are
-// staged in LDS, 128 at a time.  Per star each wavefront takes its knot
+// staged in LDS, 64 at a time.  Per star each wavefront takes its knot
 // window from the locator guesses of its first and last bin, stages the
 // window's shifted knots and spectrum values in its own LDS (no other
 // wavefront reads them) and evaluates P_j on its bins once (the per-bin
 // arithmetic of star_norm_kernel and cube_accum_kernel), then posts four
-// shares of Nn = sum_j P_j dtarget_j; after one barrier every wavefront adds
-// the 64 shares in a fixed order, forms sf = nan_to_num(T / Nn) and
+// shares of Nn = sum_j P_j dtarget_j; after one barrier (one per two stars,
+// both stars' shares posted first) every wavefront adds the 64 shares in a
+// fixed order, forms sf = nan_to_num(T / Nn) and
 // accumulates P_j sf into its f64 registers.  The shares are
 // double-buffered, so a wavefront may post the next star's share while
 // another still reads this one's.  Each (star, bin) value is computed once
