@@ -1301,7 +1301,7 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         }
         const float r = __builtin_amdgcn_rcpf(wr - wl);
         const float th = (x - wl) * r;
-        const float kap = (l0 + th * (l1 - l0)) * r;
+        const float kap = y * r;  // lambda(x) / (w_r - w_l), lambda(x) = x / a
         const float w0 = lane_of(rv.w, li), dw = lane_of(dwk, li);
         const float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
         const float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
