@@ -1279,8 +1279,18 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
         mid();
         const float y = x * h.inv_a;
         int li = k0 - wb;  // in [1, span - 3]: the guesses are monotone in the bin
-        li += lane_of(lamk, li + 1) <= y ? 1 : 0;
-        float l0 = lane_of(lamk, li), l1 = lane_of(lamk, li + 1);
+        // the next-knot test on knot k0 + 1; the segment's other knot (k0 + 2
+        // after a step, k0 otherwise) in one more permute
+        const float ln = lane_of(lamk, li + 1);
+        const bool st = ln <= y;
+        const float lo2 = lane_of(lamk, st ? li + 2 : li);
+        li += st ? 1 : 0;
+        float l0 = st ? ln : lo2, l1 = st ? lo2 : ln;
+        // the segment's row values, read before the (rare) correction vote so
+        // that their permutes overlap the knots' (read again after a correction)
+        float w0 = lane_of(rv.w, li), dw = lane_of(dwk, li);
+        float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
+        float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
         float wl = mul_nocontract(l0, h.a), wr = mul_nocontract(l1, h.a);
         if (__ballot(x >= wr || x < wl)) {  // rare: correction steps inside the window
           bool step = (x >= wr && li < span - 2) || (x < wl && li > 0);
@@ -1298,13 +1308,16 @@ __global__ void __launch_bounds__(64 * W) star_bwd_reg_kernel(
           }
           // still outside: the segment lies past the window's edge
           if (__ballot(x >= wr || x < wl)) return false;
+          w0 = lane_of(rv.w, li);
+          dw = lane_of(dwk, li);
+          u0 = lane_of(rv.u, li);
+          u1 = lane_of(rv.u, li + 1);
+          s0 = lane_of(rv.s, li);
+          s1 = lane_of(rv.s, li + 1);
         }
         const float r = __builtin_amdgcn_rcpf(wr - wl);
         const float th = (x - wl) * r;
         const float kap = y * r;  // lambda(x) / (w_r - w_l), lambda(x) = x / a
-        const float w0 = lane_of(rv.w, li), dw = lane_of(dwk, li);
-        const float u0 = lane_of(rv.u, li), u1 = lane_of(rv.u, li + 1);
-        const float s0 = lane_of(rv.s, li), s1 = lane_of(rv.s, li + 1);
         const float Ruv = u0 + th * (u1 - u0);
         const float Rsv = s0 + th * (s1 - s0);
         v[0] += gj * (w0 + th * dw);
