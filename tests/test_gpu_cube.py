@@ -211,6 +211,45 @@ def test_descending_targets_use_gather_path():
         assert e <= TOL, name
 
 
+def test_coarse_locator_runs_the_correction_paths(monkeypatch):
+    """A knot locator with cells wider than the knot spacing (forced here by a
+    small cell budget; a grid needing more than 2^23 cells takes it) holds
+    ~4 knots per cell, so the guesses are off by up to a few knots.  The
+    forward then takes its exact window searches, and the adjoint's short
+    path corrects most segments inside its 64-knot windows (reading the
+    segment's row values again after the correction) and sends the groups
+    whose segments leave the window to the general path.  Cube and all four
+    gradients against the float64 oracle, on the cfg2 grid (5000 knots) and
+    target axis (3721 ascending bins)."""
+    from oracle import ref_torch as RT
+    w = cfg2_sample(2000)
+    monkeypatch.setattr(ops.PreparedSSP, "MAX_LOCATE", 1200)
+    t, ssp, tgt, edges = _device_inputs(w)
+    assert ssp.locate_multi and ssp.n_locate < w.grid["wave"].size / 3
+    ids = ops.spaxel_assign(t["coords"], edges)
+    geom = w.geom
+    nb = geom["nb"]
+    cube, ctx = ops.cube_forward(ssp, tgt, t["mass"], t["age"], t["metallicity"], t["velocity"],
+                                 ids, nb * nb, nb * nb)
+    st = w.stars
+    f64 = torch.float64
+    leaves = [torch.tensor(st[k], dtype=f64, requires_grad=True) for k in ("mass", "age", "metallicity")]
+    vl = torch.tensor(st["velocity"][:, 2], dtype=f64, requires_grad=True)
+    ids64 = torch.tensor(R.square_spaxel_assignment(st["coords"], geom["edges"]).astype(np.int64))
+    ref = RT.star_cube(*leaves, vl, ids64, w.grid, geom, f64, True)
+    e = rel_err(cube.cpu().numpy().reshape(ref.shape), ref.detach().numpy())
+    print(f"coarse locator: cube rel64={e:.3e}")
+    assert e <= TOL
+    gen = torch.Generator(device=DEV).manual_seed(11)
+    dc = torch.randn(cube.shape, generator=gen, device=DEV)
+    got = ops.cube_backward(ssp, tgt, ctx, dc)
+    ref.backward(dc.cpu().double().reshape(ref.shape))
+    for name, a, b in zip(("mass", "age", "metallicity", "v_los"), got, leaves + [vl]):
+        e = rel_err(a.cpu().numpy(), b.grad.numpy())
+        print(f"coarse locator: d/d{name} rel64={e:.3e}")
+        assert e <= TOL, name
+
+
 def test_dense_template_grid_uses_gather_path():
     """20000 template knots (0.3 A apart): every slab window exceeds the LDS
     window, so the forward gathers per bin; checked against the oracle."""
